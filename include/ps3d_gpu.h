@@ -1,0 +1,179 @@
+/*
+ * ps3d_gpu.h -- C ABI of the MI355X-native pseudo-spectral time step.
+ *
+ * One handle owns one grid resident in the HBM of the current HIP device.
+ * All host arrays use ps3d's own field layout: Fortran (0:nz, 0:ny-1, 0:nx-1),
+ * z fastest, FP64 (src/fields/fields.f90:59-65).  Vector fields are three such
+ * volumes back to back, i.e. Fortran (0:nz, 0:ny-1, 0:nx-1, 3).
+ *
+ * Every entry point returns 0 on success and a nonzero ps3d_status on error;
+ * ps3d_gpu_last_error() then gives a message (thread-local).  The reference has
+ * no return codes: its errors stop the program through mpi_stop / mpi_exit_on_error
+ * (src/mpi/mpi_utils.f90:15-42) or a bare `stop` (src/fft/stafft.f90:86-96).  A
+ * Fortran caller maps a nonzero code onto the same mpi_stop message.
+ *
+ * A handle is single-threaded: calls on one handle must not overlap.  Each handle
+ * runs its kernels on its own HIP stream; independent handles may be used
+ * concurrently (the batched ensemble relies on this).
+ */
+#ifndef PS3D_GPU_H
+#define PS3D_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum ps3d_status {
+    PS3D_OK = 0,
+    PS3D_E_ARG = 1,         /* invalid argument / parameter                      */
+    PS3D_E_FACTOR = 2,      /* grid length not supported by the transforms       */
+    PS3D_E_HIP = 3,         /* HIP runtime error                                 */
+    PS3D_E_NOMEM = 4,       /* device allocation failed                          */
+    PS3D_E_STATE = 5,       /* call out of order (e.g. step before set_vorticity) */
+    PS3D_E_UNSUPPORTED = 6  /* option not implemented                            */
+};
+
+/* filtering (src/utils/options.f90:74): */
+enum ps3d_filtering { PS3D_FILTER_HOU_LI = 0, PS3D_FILTER_23RULE = 1 };
+/* stepper (src/utils/options.f90:24, src/ps3d.f90:90-99): */
+enum ps3d_stepper { PS3D_STEPPER_IMPL_RK4 = 0, PS3D_STEPPER_CN2 = 1 };
+/* vor_visc%pretype (src/stepper/advance.f90:381-410): */
+enum ps3d_pretype {
+    PS3D_PRE_CONSTANT = 0, PS3D_PRE_VORCH = 1, PS3D_PRE_BFMAX = 2,
+    PS3D_PRE_ROLL_MEAN_MAX_STRAIN = 3, PS3D_PRE_MAX_STRAIN = 4, PS3D_PRE_US_MAX_STRAIN = 5
+};
+/* vor_visc%length_scale (src/inversion/inversion_utils.f90:157-184): */
+enum ps3d_length_scale { PS3D_LS_KOLMOGOROV = 0, PS3D_LS_GEOPHYSICAL = 1 };
+
+/* Run parameters: the &PS3D namelist entries the time step reads
+ * (src/utils/options.f90:28-112) plus the grid box of the field file
+ * (src/utils/utils.f90:91-134) and the planetary vorticity f_cor
+ * (src/utils/physics.f90).  dt_fixed is an extension: <= 0 keeps ps3d's adaptive
+ * step (src/stepper/advance.f90:325-333); > 0 uses min(dt_fixed, limit - t). */
+typedef struct ps3d_params {
+    int nx, ny, nz;
+    double lower[3];
+    double extent[3];
+    int filtering;            /* enum ps3d_filtering                 */
+    int stepper;              /* enum ps3d_stepper                   */
+    int nnu;                  /* vor_visc%nnu                        */
+    double prediss;           /* vor_visc%prediss                    */
+    int pretype;              /* enum ps3d_pretype                   */
+    int roll_mean_win_size;   /* vor_visc%roll_mean_win_size         */
+    int length_scale;         /* enum ps3d_length_scale              */
+    double alpha;             /* time%alpha                          */
+    double limit;             /* time%limit                          */
+    double dt_fixed;          /* extension, see above                */
+    double f_cor[3];          /* planetary vorticity (physics.f90)   */
+} ps3d_params;
+
+/* Defaults of src/utils/options.f90 (filtering "Hou & Li", impl-diff-rk4,
+ * pretype roll-mean-max-strain, window 1000, Kolmogorov, alpha 0.1) with nnu=3,
+ * prediss=10 as recommended at options.f90:52-53. */
+void ps3d_params_default(ps3d_params *p);
+
+/* Which field a get/set call addresses (host layout as described above). */
+enum ps3d_field {
+    PS3D_SVOR = 0,    /* 3 volumes: mixed-spectral vorticity (state)            */
+    PS3D_VOR = 1,     /* 3 volumes: physical vorticity (after vor2vel)          */
+    PS3D_VEL = 2,     /* 3 volumes: physical velocity (after vor2vel)           */
+    PS3D_SVEL = 3,    /* 3 volumes: semi-spectral velocity (after vor2vel)      */
+    PS3D_SVORTS = 4,  /* 3 volumes: mixed-spectral tendency (after source)      */
+    PS3D_PRES = 5,    /* 1 volume : physical pressure (after derived)           */
+    PS3D_DELTA = 6    /* 1 volume : physical horizontal divergence              */
+};
+
+/* Diagnostics record written by adapt / advance: PS3D_NDIAG doubles. */
+enum ps3d_diag {
+    PS3D_D_T = 0, PS3D_D_DT, PS3D_D_KE, PS3D_D_EN, PS3D_D_HEL,
+    PS3D_D_VORTMAX, PS3D_D_VORTRMS, PS3D_D_VORCH,
+    PS3D_D_VORMEAN_X, PS3D_D_VORMEAN_Y, PS3D_D_VORMEAN_Z,
+    PS3D_D_GGMAX, PS3D_D_UMAX, PS3D_D_VMAX, PS3D_D_WMAX,
+    PS3D_D_USGGMAX, PS3D_D_LSGGMAX, PS3D_D_RGMAX, PS3D_D_PREFACTOR,
+    PS3D_NDIAG
+};
+
+/* Single operators, for kernel checks (ps3d_gpu_op). Each reads `in` and writes
+ * `out` (one volume each, host layout) on the handle's grid. */
+enum ps3d_op {
+    PS3D_OP_FFTXYP2S = 0,        /* src/fft/sta3dfft.f90:136-194                 */
+    PS3D_OP_FFTXYS2P = 1,        /* src/fft/sta3dfft.f90:202-260                 */
+    PS3D_OP_DIFFX = 2,           /* src/fft/sta3dfft.f90:304-337                 */
+    PS3D_OP_DIFFY = 3,           /* src/fft/sta3dfft.f90:345-377                 */
+    PS3D_OP_FFTSINE = 4,         /* src/fft/sta3dfft.f90:264-278                 */
+    PS3D_OP_FFTCOSINE = 5,       /* src/fft/sta3dfft.f90:282-296                 */
+    PS3D_OP_CENTRAL_DIFFZ = 6,   /* src/inversion/inversion_utils.f90:653-673    */
+    PS3D_OP_COMBINE_SEMI = 7,    /* src/inversion/inversion_utils.f90:617-647    */
+    PS3D_OP_DECOMPOSE_SEMI = 8,  /* src/inversion/inversion_utils.f90:563-592    */
+    PS3D_OP_COMBINE_PHYS = 9,    /* src/inversion/inversion_utils.f90:599-611    */
+    PS3D_OP_DECOMPOSE_PHYS = 10  /* src/inversion/inversion_utils.f90:549-557    */
+};
+
+typedef struct ps3d_handle ps3d_handle;
+
+const char *ps3d_gpu_last_error(void);
+/* Library version string. */
+const char *ps3d_gpu_version(void);
+
+/* = init_inversion + initialise_fft (inversion_utils.f90:222-372, sta3dfft.f90:53-110)
+ * on the current HIP device. Rejects lengths with a prime factor other than 2 and 3
+ * (nx, ny even; nz even) with PS3D_E_FACTOR. */
+int ps3d_gpu_create(const ps3d_params *p, ps3d_handle **h);
+/* = finalise_inversion (inversion_utils.f90:459-480). */
+int ps3d_gpu_destroy(ps3d_handle *h);
+
+/* = setup_fields (src/utils/utils.f90:136-184): decompose the physical vorticity
+ * (3 volumes) into svor, store ini_vor_mean, vor2vel, and fix the viscosity from
+ * the kinetic energy and enstrophy (init_diffusion). */
+int ps3d_gpu_set_vorticity(ps3d_handle *h, const double *vor);
+
+/* = vor2vel (src/inversion/inversion.f90:23-226) on the device state. */
+int ps3d_gpu_vor2vel(ps3d_handle *h);
+/* = source -> vorticity_tendency (src/inversion/inversion.f90:298-388). */
+int ps3d_gpu_source(ps3d_handle *h);
+/* = adapt (src/stepper/advance.f90:109-377) incl. bstep%set_diffusion; needs a
+ * preceding vor2vel.  Writes dt and a PS3D_NDIAG record (diag may be NULL). */
+int ps3d_gpu_adapt(ps3d_handle *h, double t, double *dt, double *diag);
+/* = bstep%step (impl_rk4.f90:76-207 or cn2.f90:92-181); needs preceding
+ * vor2vel + adapt + source at time *t. Advances *t. */
+int ps3d_gpu_step(ps3d_handle *h, double *t, double dt);
+/* = advance (src/stepper/advance.f90:77-104), n times, without host round trips.
+ * diag_series (may be NULL) receives n records of PS3D_NDIAG doubles. */
+int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_series);
+
+/* = pressure + horizontal_divergence (src/fields/fields_derived.f90:67-182) from the
+ * current vor2vel state; results are read with PS3D_PRES / PS3D_DELTA. */
+int ps3d_gpu_derived(ps3d_handle *h);
+
+/* Device <-> host copies of a field (enum ps3d_field). */
+int ps3d_gpu_get(ps3d_handle *h, int which, double *host);
+int ps3d_gpu_set(ps3d_handle *h, int which, const double *host);
+
+/* One operator of enum ps3d_op on host arrays (kernel checks of the reference's
+ * unit tests). */
+int ps3d_gpu_op(ps3d_handle *h, int op, const double *in, double *out);
+
+/* Batched 1-D transforms of stafft (src/fft/stafft.f90), x(m, n) column-major
+ * (m fastest), in place on a host array: kind 0 = forfft (196-287),
+ * 1 = revfft (296-403), 2 = dct on x(m, 0:n) (410-483), 3 = dst on x(m, 1:n)
+ * (489-550). Runs on the current HIP device. */
+int ps3d_gpu_fft1d(int kind, int m, int n, double *x);
+
+/* One integrating-factor RK4 substep (impl_rk4.f90:212-364, stage 1..4) applied
+ * from the stored tendency PS3D_SVORTS to PS3D_SVOR with step dt and the per-column
+ * dissipation vdiss (ny*nx doubles, ky fastest, as vdiss(ky,kx) of
+ * src/fields/fields.f90:33).  The kernel check of unit-tests/test_implicit_rk.f90,
+ * which holds the tendency fixed across the four stages. */
+int ps3d_gpu_rk_substep(ps3d_handle *h, int stage, double dt, const double *vdiss);
+
+/* Wait for all work queued on the handle's stream. */
+int ps3d_gpu_sync(ps3d_handle *h);
+
+/* Kernel time of the most recent advance_n call, measured with HIP events on the
+ * handle's stream: total milliseconds and number of steps. */
+int ps3d_gpu_last_timing(ps3d_handle *h, double *ms, int *nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PS3D_GPU_H */

@@ -1,0 +1,273 @@
+"""ctypes bindings of the CPU oracle (liboracle.so) and of the reference's own
+Fortran transform core compiled into oracle/_ref (libps3d_ref_core.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module as the checker; the product package never
+imports it.
+
+Arrays use ps3d's field layout (src/fields/fields.f90:59-65): numpy shape
+(nx, ny, nz+1), C order, i.e. Fortran (0:nz, 0:ny-1, 0:nx-1); vector fields
+(3, nx, ny, nz+1).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(HERE, "liboracle.so")
+_REF = os.path.join(HERE, "_ref", "libps3d_ref_core.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+# field / op / diag enums of include/ps3d_gpu.h
+SVOR, VOR, VEL, SVEL, SVORTS, PRES, DELTA = range(7)
+(OP_FFTXYP2S, OP_FFTXYS2P, OP_DIFFX, OP_DIFFY, OP_FFTSINE, OP_FFTCOSINE,
+ OP_CENTRAL_DIFFZ, OP_COMBINE_SEMI, OP_DECOMPOSE_SEMI, OP_COMBINE_PHYS,
+ OP_DECOMPOSE_PHYS) = range(11)
+NDIAG = 19
+DIAG_NAMES = ["t", "dt", "ke", "en", "hel", "vortmax", "vortrms", "vorch",
+              "vormean_x", "vormean_y", "vormean_z", "ggmax", "umax", "vmax",
+              "wmax", "usggmax", "lsggmax", "rgmax", "prefactor"]
+
+
+class Params(C.Structure):
+    """Mirror of ps3d_params (include/ps3d_gpu.h)."""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("lower", C.c_double * 3), ("extent", C.c_double * 3),
+                ("filtering", C.c_int), ("stepper", C.c_int), ("nnu", C.c_int),
+                ("prediss", C.c_double), ("pretype", C.c_int),
+                ("roll_mean_win_size", C.c_int), ("length_scale", C.c_int),
+                ("alpha", C.c_double), ("limit", C.c_double),
+                ("dt_fixed", C.c_double), ("f_cor", C.c_double * 3)]
+
+
+FILTERING = {"Hou & Li": 0, "2/3-rule": 1}
+STEPPER = {"impl-diff-rk4": 0, "cn2": 1}
+PRETYPE = {"constant": 0, "vorch": 1, "bfmax": 2, "roll-mean-max-strain": 3,
+           "max-strain": 4, "us-max-strain": 5}
+LENGTH_SCALE = {"Kolmogorov": 0, "geophysical": 1}
+
+
+def make_params(nx, ny, nz, lower=(0.0, 0.0, 0.0), extent=(1.0, 1.0, 1.0),
+                filtering="Hou & Li", stepper="impl-diff-rk4", nnu=3,
+                prediss=10.0, pretype="roll-mean-max-strain",
+                roll_mean_win_size=1000, length_scale="Kolmogorov", alpha=0.1,
+                limit=1.0e30, dt_fixed=0.0, f_cor=(0.0, 0.0, 0.0)):
+    p = Params()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.lower[:] = list(lower)
+    p.extent[:] = list(extent)
+    p.filtering = FILTERING.get(filtering, filtering)
+    p.stepper = STEPPER.get(stepper, stepper)
+    p.nnu = nnu
+    p.prediss = prediss
+    p.pretype = PRETYPE.get(pretype, pretype)
+    p.roll_mean_win_size = roll_mean_win_size
+    p.length_scale = LENGTH_SCALE.get(length_scale, length_scale)
+    p.alpha = alpha
+    p.limit = limit
+    p.dt_fixed = dt_fixed
+    p.f_cor[:] = list(f_cor)
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(_LIB)
+        for name in ("orc_forfft", "orc_revfft", "orc_dct", "orc_dst"):
+            getattr(L, name).argtypes = [C.c_int, C.c_int, _dp]
+            getattr(L, name).restype = C.c_int
+        L.orc_jacobi_eigenvalues.argtypes = [_dp, _dp]
+        L.orc_create.argtypes = [C.POINTER(Params)]
+        L.orc_create.restype = C.c_void_p
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_op.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.orc_set_vorticity.argtypes = [C.c_void_p, _dp]
+        L.orc_vor2vel.argtypes = [C.c_void_p]
+        L.orc_source.argtypes = [C.c_void_p]
+        L.orc_adapt.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_void_p]
+        L.orc_step.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double]
+        L.orc_advance_n.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_void_p]
+        L.orc_derived.argtypes = [C.c_void_p]
+        L.orc_rk4_check.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
+        L.orc_get.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_set.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_table.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_table.restype = C.c_long
+        L.orc_viscosity.argtypes = [C.c_void_p]
+        L.orc_viscosity.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        raise RuntimeError("oracle error %d: %s" % (st, lib().orc_last_error().decode()))
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def fft1d(kind, x):
+    """stafft transform of x with shape (n_or_n+1, m) viewed as Fortran x(m, n).
+
+    x is given as a numpy array of shape (len, m) in C order, which is the memory
+    of Fortran x(m, len). kind: 'forfft', 'revfft', 'dct', 'dst'."""
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    ln, m = x.shape
+    n = ln - 1 if kind == "dct" else ln
+    f = {"forfft": lib().orc_forfft, "revfft": lib().orc_revfft,
+         "dct": lib().orc_dct, "dst": lib().orc_dst}[kind]
+    _check(f(m, n, x))
+    return x
+
+
+def jacobi_eigenvalues(a):
+    a = np.asfortranarray(np.array(a, dtype=np.float64)).ravel(order="F").copy()
+    d = np.zeros(3)
+    lib().orc_jacobi_eigenvalues(a, d)
+    return d
+
+
+class Oracle:
+    """CPU restatement of ps3d's solver state and operators."""
+
+    def __init__(self, params):
+        self.p = params
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self.shape = (self.nx, self.ny, self.nz + 1)
+        self.h = lib().orc_create(C.byref(params))
+        if not self.h:
+            raise RuntimeError("oracle create failed: " + lib().orc_last_error().decode())
+        self.t = 0.0
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def op(self, op, fin):
+        fin = np.ascontiguousarray(fin, dtype=np.float64)
+        out = np.zeros(self.shape)
+        _check(lib().orc_op(self.h, op, fin.reshape(self.shape).copy(), out))
+        return out
+
+    def set_vorticity(self, vor):
+        _check(lib().orc_set_vorticity(self.h, np.ascontiguousarray(vor, dtype=np.float64)))
+
+    def vor2vel(self):
+        _check(lib().orc_vor2vel(self.h))
+
+    def source(self):
+        _check(lib().orc_source(self.h))
+
+    def adapt(self, t):
+        dt = C.c_double()
+        diag = np.zeros(NDIAG)
+        _check(lib().orc_adapt(self.h, t, C.byref(dt), diag.ctypes.data))
+        return dt.value, diag
+
+    def step(self, t, dt):
+        tt = C.c_double(t)
+        _check(lib().orc_step(self.h, C.byref(tt), dt))
+        return tt.value
+
+    def advance_n(self, nsteps):
+        tt = C.c_double(self.t)
+        diag = np.zeros((nsteps, NDIAG))
+        _check(lib().orc_advance_n(self.h, C.byref(tt), nsteps, diag.ctypes.data))
+        self.t = tt.value
+        return diag
+
+    def derived(self):
+        _check(lib().orc_derived(self.h))
+
+    def rk4_check(self, dt, vdiss, src, svor):
+        svor = np.ascontiguousarray(svor, dtype=np.float64).copy()
+        _check(lib().orc_rk4_check(self.h, dt, np.ascontiguousarray(vdiss, dtype=np.float64),
+                                   np.ascontiguousarray(src, dtype=np.float64), svor))
+        return svor
+
+    def get(self, which):
+        nv = 1 if which in (PRES, DELTA) else 3
+        out = np.zeros((nv,) + self.shape) if nv == 3 else np.zeros(self.shape)
+        _check(lib().orc_get(self.h, which, out))
+        return out
+
+    def set(self, which, arr):
+        _check(lib().orc_set(self.h, which, np.ascontiguousarray(arr, dtype=np.float64)))
+
+    def table(self, which):
+        n = lib().orc_table(self.h, which, None)
+        out = np.zeros(n)
+        lib().orc_table(self.h, which, out.ctypes.data)
+        return out
+
+    def viscosity(self):
+        return lib().orc_viscosity(self.h)
+
+
+# --------------------------------------------------------------------------
+# The reference's own compiled transform core (oracle/_ref)
+# --------------------------------------------------------------------------
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(_REF)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(_REF)
+    return _ref
+
+
+def _ip(v):
+    return C.byref(C.c_int(v))
+
+
+def ref_fft1d(kind, x):
+    """Same contract as fft1d() but through the reference's stafft.f90."""
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    ln, m = x.shape
+    n = ln - 1 if kind == "dct" else ln
+    trig = np.zeros(2 * n)
+    fac = np.zeros(5, dtype=np.int32)
+    R = ref()
+    R._QMstafftPinitfft(_ip(n), fac.ctypes.data_as(C.c_void_p), trig.ctypes.data_as(C.c_void_p))
+    f = {"forfft": R._QMstafftPforfft, "revfft": R._QMstafftPrevfft,
+         "dct": R._QMstafftPdct, "dst": R._QMstafftPdst}[kind]
+    f(_ip(m), _ip(n), x.ctypes.data_as(C.c_void_p), trig.ctypes.data_as(C.c_void_p),
+      fac.ctypes.data_as(C.c_void_p))
+    return x
+
+
+def ref_jacobi_eigenvalues(a):
+    a = np.asfortranarray(np.array(a, dtype=np.float64)).ravel(order="F").copy()
+    d = np.zeros(3)
+    ref()._QMjacobiPjacobi_eigenvalues(a.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p))
+    return d
+
+
+def ref_init_deriv(n, length):
+    k = np.zeros(n)
+    ref()._QMderiv1dPinit_deriv(_ip(n), C.byref(C.c_double(length)), k.ctypes.data_as(C.c_void_p))
+    return k

@@ -1,0 +1,167 @@
+// kernels.hpp -- kernel argument blocks and launchers shared by the passes.
+#pragma once
+#include "common.hpp"
+
+namespace ps3d {
+
+// Device-resident scalars (one small array per handle).  Kernels read dt and the
+// diffusion factor from here, so a whole step can run (and be captured in a HIP
+// graph) without host round trips.
+enum Scal {
+    SC_T = 0,        // time
+    SC_DT,           // current step
+    SC_DFAC,         // impl-rk4: pref*dt/2 ; cn2: dt or pref*dt (impl_rk4.f90:37-53, cn2.f90:39-80)
+    SC_VISC,         // vvisc (inversion_utils.f90:157-184)
+    SC_INI_XI,       // ini_vor_mean(1:2) (utils.f90:176)
+    SC_INI_ETA,
+    SC_VORTRMS,      // sqrt(<|omega|^2>)
+    SC_KE, SC_EN, SC_HEL,
+    SC_VORMX, SC_VORMY, SC_VORMZ,
+    SC_PREF,
+    SC_RM_SMA,       // rolling mean state (rolling_mean.f90)
+    SC_RM_INEW, SC_RM_IOLD, SC_RM_FILLED,
+    SC_STEP,         // step counter (diag series row)
+    SC_NSCAL = 32
+};
+
+// z-kernel modes
+enum { ZT_STORE = 0, ZT_RK4 = 1, ZT_CN2 = 2 };
+enum { ZOP_DST = 0, ZOP_DCT = 1, ZOP_DIFFZ = 2, ZOP_COMBINE = 3, ZOP_DECOMPOSE = 4 };
+enum { MEAN_CALC = 0, MEAN_ADJUST = 1 };
+
+// 1-D tables on the device
+struct ZTabs {
+    const double *rkx, *rky;      // wavenumbers (sta3dfft.f90:86-104)
+    const double *filx, *fily, *filz;   // filter factors: Hou&Li exponents or 2/3 0/1 (inversion_utils.f90:377-455)
+    int hou_li;
+    int nnu;
+};
+
+struct ZLaunch {
+    int zpt, blocks;
+    size_t lds;
+};
+
+struct ZInvArgs {
+    Grid g;
+    ZTabs tb;
+    int TQ, QPB, nquads;
+    FftPlan pz;
+    const double2 *twz;
+    double zscale;     // 1/sqrt(2 nz)
+    double rkz_sc;     // pi / Lz (deriv1d.f90:20-23)
+    const double *svor[3];
+    double *svor_out[2];
+    double *out[6];
+    double *mid[3];
+};
+
+struct RkArgs {
+    double *q[3], *qdi[3], *qdf[3];
+    double *scal;
+    const double *vdiss;   // optional per-column dissipation (ky fastest); nullptr: dfac*vhdis
+};
+
+struct ZTendArgs {
+    Grid g;
+    ZTabs tb;
+    int TQ, QPB, nquads;
+    FftPlan pz;
+    const double2 *twz;
+    double zscale;
+    const double *rqp[3];
+    double *svorts[3];
+    int mode, stage;
+    RkArgs rk;
+};
+
+struct ZOpArgs {
+    Grid g;
+    ZTabs tb;
+    int TQ, QPB, nquads;
+    FftPlan pz;
+    const double2 *twz;
+    double zscale;
+    const double *in;
+    double *out;
+    int op;
+};
+
+struct MeanArgs {
+    Grid g;
+    FftPlan pz;
+    const double2 *twz;
+    double zscale;
+    double *svor[2];
+    double *scal;
+    int mode, advance_time;
+};
+
+size_t zpass_lds_bytes(int QPB, int nz);
+hipError_t launch_zinv(const ZLaunch &zl, const ZInvArgs &args, hipStream_t stream);
+hipError_t launch_ztend(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream);
+hipError_t launch_zrk(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream);
+hipError_t launch_zop(const ZLaunch &zl, const ZOpArgs &args, hipStream_t stream);
+hipError_t launch_mean(const MeanArgs &args, int nz, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
+// x / y line passes
+// ---------------------------------------------------------------------------
+enum LineMode {
+    LM_FWD = 0,       // forward Hermitian transform of nf fields
+    LM_INV = 1,       // inverse transform of nf fields
+    LM_INV_Y1 = 2,    // y inverse at the adapt stage: u,v,w,xi,eta,zeta(,mid) + y-derivatives + Parseval sums
+    LM_XMAIN = 3,     // x: inverse 6 fields, u x omega, forward 3
+    LM_XADAPT = 4,    // x: strain eigenvalues, maxima, characteristic vorticity
+    LM_DIFF = 5       // spectral derivative along the line (diffx / diffy), no transform
+};
+
+// Reduction slots written per block by LM_INV_Y1 (sums) and LM_XADAPT (maxima / sums)
+enum { RY_UU = 0, RY_WW, RY_UW, RY_MX, RY_MY, RY_MZ, RY_N };
+enum { RX_W2MAX = 0, RX_GG, RX_USGG, RX_LSGG, RX_UMAX, RX_VMAX, RX_WMAX, RX_L1, RX_L2, RX_N };
+
+struct LineArgs {
+    Grid g;
+    int axis;          // 0: lines along x, 1: lines along y
+    int n;             // line length
+    long estride;      // element stride along the line (doubles)
+    long ostride;      // stride of the "other" horizontal index (doubles)
+    int nother;        // number of lines in the other horizontal direction
+    int ZC, nzt;       // z depth of a tile, number of z tiles
+    FftPlan P;
+    const double2 *tw; // exp(-2 pi i m/n)
+    const double *rk;  // wavenumber table along the line
+    int nwl;           // n/2
+    int mode, nf;
+    int phys_in;       // LM_XADAPT / LM_XMAIN: inputs already physical in x
+    const double *in[12];
+    double *out[12];
+    double *part;      // per-block partial results
+    const double *scal;
+    double fcor[3];
+    double ncelli;
+    double sqrt_nx;    // for means from the kx = 0 slot
+};
+
+size_t line_lds_bytes(int n, int ZC);
+hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream);
+
+// Reductions: finishes the Parseval sums (after LM_INV_Y1) and the adapt step
+// (after LM_XADAPT): dt, prefactor, rolling mean, diffusion factor, diag row.
+struct FinArgs {
+    const double *party;  int nby;
+    const double *partx;  int nbx;
+    double *scal;
+    double *diag;         // series base (row SC_STEP), or nullptr
+    int max_rows;
+    double ncelli;
+    double alpha, limit, dt_fixed;
+    double dx, dy, dz;
+    int pretype, stepper, nnu;
+    int roll_len;
+    double *roll_hist;
+};
+hipError_t launch_fin_y(const FinArgs &a, hipStream_t stream);
+hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream);
+
+}  // namespace ps3d

@@ -1,0 +1,639 @@
+// linepass.hip -- horizontal (x or y) line passes of the MI355X pseudo-spectral step.
+//
+// A block owns a tile of n line elements x ZC z levels at one fixed index of the
+// other horizontal direction.  In the column layout the tile is read as n runs of
+// ZC consecutive doubles (128 B for ZC = 16).  Levels 2p and 2p+1 share one
+// complex line (re, im), so one complex FFT of length n gives the real transforms
+// of two levels; the Hermitian real form of stafft (src/fft/stafft.f90:55-58,
+// 196-403) keeps the same packing: slot j of line p holds (level 2p, level 2p+1).
+#include "kernels.hpp"
+
+namespace ps3d {
+
+constexpr int kEpt = 4;        // complex elements per thread held in registers
+constexpr int kLineThreads = 512;
+
+struct Tile {
+    int n, ZC, nl, LS, zce, z0, other;
+    long base;
+};
+
+__device__ inline Tile make_tile(const LineArgs &a) {
+    Tile t;
+    t.n = a.n;
+    t.ZC = a.ZC;
+    t.nl = a.ZC / 2;
+    t.LS = a.n + 1;
+    const int zt = blockIdx.x % a.nzt;
+    t.other = blockIdx.x / a.nzt;
+    t.z0 = zt * a.ZC;
+    const int rem = a.g.nz + 1 - t.z0;
+    t.zce = rem < a.ZC ? rem : a.ZC;
+    t.base = (long)t.other * a.ostride + t.z0;
+    return t;
+}
+
+__device__ inline void tile_load(const Tile &t, const LineArgs &a, const double *__restrict__ src, double2 *buf) {
+    double *d = reinterpret_cast<double *>(buf);
+    const int tot = t.n * t.ZC;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int zz = e % t.ZC, j = e / t.ZC;
+        const double v = zz < t.zce ? src[t.base + (long)j * a.estride + zz] : 0.0;
+        d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = v;
+    }
+}
+
+__device__ inline void tile_store(const Tile &t, const LineArgs &a, double *__restrict__ dst, const double2 *buf,
+                                  double scale) {
+    const double *d = reinterpret_cast<const double *>(buf);
+    const int tot = t.n * t.ZC;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int zz = e % t.ZC, j = e / t.ZC;
+        if (zz < t.zce) dst[t.base + (long)j * a.estride + zz] = d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] * scale;
+    }
+}
+
+// Hermitian pair -> complex spectrum, then inverse FFT.  Returns result buffer,
+// values to be scaled by 1/n.
+__device__ inline double2 *tile_inverse(const Tile &t, const LineArgs &a, double2 *A, double2 *B) {
+    const int n = t.n, h = n / 2;
+    const double sn = sqrt((double)n);
+    const int tot = t.nl * (h + 1);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int line = e / (h + 1), k = e % (h + 1);
+        double2 *L = A + (size_t)line * t.LS;
+        if (k == 0 || k == h) {
+            const double2 v = L[k];
+            L[k] = make_double2(sn * v.x, sn * v.y);
+        } else {
+            const double2 ck = L[k], sk = L[n - k];   // (a(k), b(k)), (a(n-k), b(n-k))
+            L[k] = make_double2(sn * (ck.x - sk.y), sn * (sk.x + ck.y));
+            L[n - k] = make_double2(sn * (ck.x + sk.y), sn * (-sk.x + ck.y));
+        }
+    }
+    __syncthreads();
+    return fft_lds(A, B, t.nl, t.LS, a.P, a.tw, +1);
+}
+
+// Forward FFT then split into the Hermitian pair, in the returned buffer, scaled.
+__device__ inline double2 *tile_forward(const Tile &t, const LineArgs &a, double2 *A, double2 *B) {
+    double2 *R = fft_lds(A, B, t.nl, t.LS, a.P, a.tw, -1);
+    const int n = t.n, h = n / 2;
+    const double sc = 1.0 / sqrt((double)n);
+    const int tot = t.nl * (h + 1);
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int line = e / (h + 1), k = e % (h + 1);
+        double2 *L = R + (size_t)line * t.LS;
+        const int nk = (n - k) % n;
+        const double2 zk = L[k], znk = L[nk];
+        const double Ar = 0.5 * (zk.x + znk.x), Ai = 0.5 * (zk.y - znk.y);
+        const double Br = 0.5 * (zk.y + znk.y), Bi = -0.5 * (zk.x - znk.x);
+        L[k] = make_double2(Ar * sc, Br * sc);
+        if (k != 0 && k != h) L[nk] = make_double2(Ai * sc, Bi * sc);
+    }
+    __syncthreads();
+    return R;
+}
+
+// Spectral derivative along the line (diffx / diffy, sta3dfft.f90:304-377):
+// dst[j] = si(j) rk(j) src[n-j], zero at j = 0 and j = n/2.
+__device__ inline void tile_diff(const Tile &t, const LineArgs &a, const double2 *src, double2 *dst) {
+    const int n = t.n, h = n / 2;
+    const int tot = t.nl * n;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int line = e / n, j = e % n;
+        double f = (j >= h + 1) ? a.rk[j] : -a.rk[j];
+        if (j == 0 || j == h) f = 0.0;
+        const double2 v = src[(size_t)line * t.LS + (n - j) % n];
+        dst[(size_t)line * t.LS + j] = make_double2(f * v.x, f * v.y);
+    }
+    __syncthreads();
+}
+
+// registers <-> tile
+__device__ inline void to_regs(const Tile &t, const double2 *buf, double2 (&r)[kEpt], double scale) {
+    const int tot = t.nl * t.n;
+#pragma unroll
+    for (int k = 0; k < kEpt; ++k) {
+        const int e = threadIdx.x + k * blockDim.x;
+        if (e < tot) {
+            const int line = e / t.n, j = e % t.n;
+            const double2 v = buf[(size_t)line * t.LS + j];
+            r[k] = make_double2(v.x * scale, v.y * scale);
+        } else {
+            r[k] = make_double2(0.0, 0.0);
+        }
+    }
+}
+__device__ inline void from_regs(const Tile &t, double2 *buf, const double2 (&r)[kEpt]) {
+    const int tot = t.nl * t.n;
+#pragma unroll
+    for (int k = 0; k < kEpt; ++k) {
+        const int e = threadIdx.x + k * blockDim.x;
+        if (e < tot) {
+            const int line = e / t.n, j = e % t.n;
+            buf[(size_t)line * t.LS + j] = r[k];
+        }
+    }
+}
+// z level (absolute) of component c (0 = re, 1 = im) of register element k
+__device__ inline int reg_z(const Tile &t, int k, int c) {
+    const int e = threadIdx.x + k * blockDim.x;
+    const int line = e / t.n;
+    return t.z0 + 2 * line + c;
+}
+__device__ inline bool reg_valid(const Tile &t, int k, int c, int nz) {
+    const int e = threadIdx.x + k * blockDim.x;
+    if (e >= t.nl * t.n) return false;
+    const int zz = 2 * (e / t.n) + c;
+    return zz < t.zce && t.z0 + zz <= nz;
+}
+__device__ inline int reg_j(const Tile &t, int k) { return (threadIdx.x + k * blockDim.x) % t.n; }
+
+// Load one field, transform it to physical space along the line (unless the input
+// is already physical), and keep it in registers.
+__device__ inline void field_phys(const Tile &t, const LineArgs &a, const double *src, double2 *A, double2 *B,
+                                  double2 (&r)[kEpt], bool diff) {
+    tile_load(t, a, src, A);
+    __syncthreads();
+    double2 *S = A, *O = B;
+    if (diff) {
+        if (a.phys_in) {
+            S = tile_forward(t, a, A, B);
+            O = (S == A) ? B : A;
+        }
+        tile_diff(t, a, S, O);
+        double2 *tmp = S;
+        S = O;
+        O = tmp;
+    } else if (a.phys_in) {
+        to_regs(t, A, r, 1.0);
+        __syncthreads();
+        return;
+    }
+    double2 *R = tile_inverse(t, a, S, O);
+    to_regs(t, R, r, 1.0 / (double)t.n);
+    __syncthreads();
+}
+
+// registers -> forward transform -> global
+__device__ inline void fwd_store(const Tile &t, const LineArgs &a, double2 *A, double2 *B, const double2 (&r)[kEpt],
+                                 double *dst) {
+    from_regs(t, A, r);
+    __syncthreads();
+    double2 *R = tile_forward(t, a, A, B);
+    tile_store(t, a, dst, R, 1.0);
+    __syncthreads();
+}
+
+// block-wide reductions (deterministic tree in LDS)
+__device__ inline double block_sum(double v, double *scratch) {
+    const int tid = threadIdx.x;
+    scratch[tid] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) scratch[tid] += scratch[tid + w];
+        __syncthreads();
+    }
+    const double r = scratch[0];
+    __syncthreads();
+    return r;
+}
+__device__ inline double block_max(double v, double *scratch) {
+    const int tid = threadIdx.x;
+    scratch[tid] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) scratch[tid] = fmax(scratch[tid], scratch[tid + w]);
+        __syncthreads();
+    }
+    const double r = scratch[0];
+    __syncthreads();
+    return r;
+}
+
+// Cyclic Jacobi eigenvalues of a symmetric 3x3 matrix (Rutishauser 1966, as in
+// src/utils/jacobi.f90:20-305): returns max |lambda|.  Bounded number of sweeps.
+__device__ inline void jac_rot(double (&A)[3][3], double (&D)[3], double (&Z)[3], int i, int j) {
+    const double eps = 2.220446049250313e-16;
+    const double aij = A[i][j];
+    double g = 100.0 * fabs(aij);
+    double h = D[j] - D[i];
+    double t;
+    if (fabs(h) + g == fabs(h)) {
+        t = aij / (h + copysign(eps, h));
+    } else {
+        const double theta = 0.5 * h / (aij + copysign(eps, aij));
+        t = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+        if (theta < 0.0) t = -t;
+    }
+    const double c = 1.0 / sqrt(1.0 + t * t);
+    const double s = t * c;
+    const double tau = s / (1.0 + c);
+    h = t * aij;
+    Z[i] -= h;
+    Z[j] += h;
+    D[i] -= h;
+    D[j] += h;
+    A[i][j] = 0.0;
+    for (int k = 0; k < i; ++k) {
+        g = A[k][i]; h = A[k][j];
+        A[k][i] = g - s * (h + g * tau);
+        A[k][j] = h + s * (g - h * tau);
+    }
+    for (int k = i + 1; k < j; ++k) {
+        g = A[i][k]; h = A[k][j];
+        A[i][k] = g - s * (h + g * tau);
+        A[k][j] = h + s * (g - h * tau);
+    }
+    for (int k = j + 1; k < 3; ++k) {
+        g = A[i][k]; h = A[j][k];
+        A[i][k] = g - s * (h + g * tau);
+        A[j][k] = h + s * (g - h * tau);
+    }
+}
+__device__ inline double jacobi_absmax(double s11, double s12, double s13, double s22, double s23, double s33) {
+    double A[3][3] = {{s11, s12, s13}, {0.0, s22, s23}, {0.0, 0.0, s33}};
+    double D[3] = {s11, s22, s33}, B[3] = {s11, s22, s33}, Z[3] = {0.0, 0.0, 0.0};
+    double sm = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+    for (int sweep = 0; sweep < 64 && sm > 1.0e-15; ++sweep) {
+        jac_rot(A, D, Z, 0, 1);
+        jac_rot(A, D, Z, 0, 2);
+        jac_rot(A, D, Z, 1, 2);
+        for (int k = 0; k < 3; ++k) {
+            B[k] += Z[k];
+            D[k] = B[k];
+            Z[k] = 0.0;
+        }
+        sm = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
+    }
+    return fmax(fmax(fabs(D[0]), fabs(D[1])), fabs(D[2]));
+}
+
+__global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
+    extern __shared__ double2 lsmem[];
+    const Tile t = make_tile(a);
+    const size_t tb = (size_t)t.nl * t.LS;
+    double2 *A = lsmem, *B = lsmem + tb;
+    double *scratch = reinterpret_cast<double *>(lsmem);
+    const int nz = a.g.nz;
+
+    if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF) {
+        for (int f = 0; f < a.nf; ++f) {
+            tile_load(t, a, a.in[f], A);
+            __syncthreads();
+            if (a.mode == LM_FWD) {
+                double2 *R = tile_forward(t, a, A, B);
+                tile_store(t, a, a.out[f], R, 1.0);
+            } else if (a.mode == LM_INV) {
+                double2 *R = tile_inverse(t, a, A, B);
+                tile_store(t, a, a.out[f], R, 1.0 / (double)t.n);
+            } else {
+                tile_diff(t, a, A, B);
+                tile_store(t, a, a.out[f], B, 1.0);
+            }
+            __syncthreads();
+        }
+        return;
+    }
+
+    if (a.mode == LM_INV_Y1) {
+        // u, v, w with their y derivatives; xi, eta, zeta; optional midpoints.
+        // Parseval in x (this tile is one x slot): sum_x f^2 = sum_k w(k) f_k^2,
+        // w = 1 at k = 0, nx/2 and 2 elsewhere.
+        const int kx = t.other;
+        const double wk = (kx == 0 || kx == a.g.nx / 2) ? 1.0 : 2.0;
+        double2 U[3][kEpt];
+        double suu = 0.0, sww = 0.0, suw = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            tile_load(t, a, a.in[c], A);
+            __syncthreads();
+            tile_diff(t, a, A, B);            // y derivative in B
+            double2 *R = tile_inverse(t, a, B, A);
+            tile_store(t, a, a.out[9 + c], R, 1.0 / (double)t.n);
+            __syncthreads();
+            tile_load(t, a, a.in[c], A);
+            __syncthreads();
+            R = tile_inverse(t, a, A, B);
+            tile_store(t, a, a.out[c], R, 1.0 / (double)t.n);
+            to_regs(t, R, U[c], 1.0 / (double)t.n);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            tile_load(t, a, a.in[3 + c], A);
+            __syncthreads();
+            double2 *R = tile_inverse(t, a, A, B);
+            tile_store(t, a, a.out[3 + c], R, 1.0 / (double)t.n);
+            double2 W[kEpt];
+            to_regs(t, R, W, 1.0 / (double)t.n);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kEpt; ++k)
+                for (int h = 0; h < 2; ++h) {
+                    if (!reg_valid(t, k, h, nz)) continue;
+                    const int z = reg_z(t, k, h);
+                    const double wz = (z == 0 || z == nz) ? 0.5 : 1.0;
+                    const double w = h ? W[k].y : W[k].x;
+                    const double u = h ? U[c][k].y : U[c][k].x;
+                    suu += wz * wk * u * u;
+                    sww += wz * wk * w * w;
+                    suw += wz * wk * u * w;
+                    if (kx == 0) {
+                        if (c == 0) m0 += wz * w;
+                        if (c == 1) m1 += wz * w;
+                        if (c == 2) m2 += wz * w;
+                    }
+                }
+        }
+        if (a.in[6]) {
+            for (int c = 0; c < 3; ++c) {
+                tile_load(t, a, a.in[6 + c], A);
+                __syncthreads();
+                double2 *R = tile_inverse(t, a, A, B);
+                tile_store(t, a, a.out[6 + c], R, 1.0 / (double)t.n);
+                __syncthreads();
+            }
+        }
+        suu = block_sum(suu, scratch);
+        sww = block_sum(sww, scratch);
+        suw = block_sum(suw, scratch);
+        m0 = block_sum(m0, scratch);
+        m1 = block_sum(m1, scratch);
+        m2 = block_sum(m2, scratch);
+        if (threadIdx.x == 0) {
+            double *p = a.part + (size_t)blockIdx.x * RY_N;
+            p[RY_UU] = suu;
+            p[RY_WW] = sww;
+            p[RY_UW] = suw;
+            p[RY_MX] = m0 * a.sqrt_nx;
+            p[RY_MY] = m1 * a.sqrt_nx;
+            p[RY_MZ] = m2 * a.sqrt_nx;
+        }
+        return;
+    }
+
+    if (a.mode == LM_XMAIN) {
+        // u x omega in flux form (inversion.f90:316-352) with omega + f_cor
+        double2 u[kEpt], v[kEpt], w[kEpt], r[kEpt], q[kEpt], p[kEpt];
+        field_phys(t, a, a.in[0], A, B, u, false);
+        field_phys(t, a, a.in[1], A, B, v, false);
+        field_phys(t, a, a.in[2], A, B, w, false);
+        double2 o[kEpt];
+        field_phys(t, a, a.in[3], A, B, o, false);   // xi
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k) {
+            const double xr = o[k].x + a.fcor[0], xi = o[k].y + a.fcor[0];
+            r[k] = make_double2(-v[k].x * xr, -v[k].y * xi);
+            q[k] = make_double2(w[k].x * xr, w[k].y * xi);
+        }
+        field_phys(t, a, a.in[4], A, B, o, false);   // eta
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k) {
+            const double er = o[k].x + a.fcor[1], ei = o[k].y + a.fcor[1];
+            r[k] = make_double2(u[k].x * er + r[k].x, u[k].y * ei + r[k].y);
+            p[k] = make_double2(-w[k].x * er, -w[k].y * ei);
+        }
+        field_phys(t, a, a.in[5], A, B, o, false);   // zeta
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k) {
+            const double zr = o[k].x + a.fcor[2], zi = o[k].y + a.fcor[2];
+            q[k] = make_double2(q[k].x - u[k].x * zr, q[k].y - u[k].y * zi);
+            p[k] = make_double2(v[k].x * zr + p[k].x, v[k].y * zi + p[k].y);
+        }
+        // r = u eta - v xi ; q = w xi - u zeta ; p = v zeta - w eta  -> forward in x
+        fwd_store(t, a, A, B, r, a.out[0]);
+        fwd_store(t, a, A, B, q, a.out[1]);
+        fwd_store(t, a, A, B, p, a.out[2]);
+        return;
+    }
+
+    if (a.mode == LM_XADAPT) {
+        // adapt (advance.f90:171-289): max |omega|, signed max of u, v, w, largest
+        // |eigenvalue| of the symmetrised strain (Jacobi per point), lid maxima, and
+        // the sums of the characteristic vorticity.
+        const double vortrms = a.scal[SC_VORTRMS];
+        double umax = -INFINITY, vmax = -INFINITY, wmax = -INFINITY, w2max = 0.0;
+        double gg = 2.220446049250313e-16, usgg = 0.0, lsgg = 0.0, l1 = 0.0, l2 = 0.0;
+        double2 ux[kEpt], uy[kEpt], vy[kEpt], wx[kEpt], wy[kEpt], xi[kEpt], et[kEpt], ze[kEpt];
+        field_phys(t, a, a.in[0], A, B, ux, false);   // u (for umax), then du/dx
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k)
+            for (int h = 0; h < 2; ++h)
+                if (reg_valid(t, k, h, nz)) umax = fmax(umax, h ? ux[k].y : ux[k].x);
+        field_phys(t, a, a.in[1], A, B, vy, false);
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k)
+            for (int h = 0; h < 2; ++h)
+                if (reg_valid(t, k, h, nz)) vmax = fmax(vmax, h ? vy[k].y : vy[k].x);
+        field_phys(t, a, a.in[2], A, B, wx, false);
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k)
+            for (int h = 0; h < 2; ++h)
+                if (reg_valid(t, k, h, nz)) wmax = fmax(wmax, h ? wx[k].y : wx[k].x);
+        field_phys(t, a, a.in[0], A, B, ux, true);    // du/dx
+        field_phys(t, a, a.in[2], A, B, wx, true);    // dw/dx
+        field_phys(t, a, a.in[6], A, B, uy, false);   // du/dy
+        field_phys(t, a, a.in[7], A, B, vy, false);   // dv/dy
+        field_phys(t, a, a.in[8], A, B, wy, false);   // dw/dy
+        field_phys(t, a, a.in[3], A, B, xi, false);
+        field_phys(t, a, a.in[4], A, B, et, false);
+        field_phys(t, a, a.in[5], A, B, ze, false);
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k)
+            for (int h = 0; h < 2; ++h) {
+                if (!reg_valid(t, k, h, nz)) continue;
+                const int z = reg_z(t, k, h);
+#define C_(V) (h ? (V).y : (V).x)
+                const double o1 = C_(xi[k]), o2 = C_(et[k]), o3 = C_(ze[k]);
+                w2max = fmax(w2max, o1 * o1 + o2 * o2 + o3 * o3);
+                // S11 = u_x, S12 = u_y + zeta/2, S13 = w_x + eta/2, S22 = v_y,
+                // S23 = w_y - xi/2, S33 = -(u_x + v_y)  (advance.f90:225-257)
+                const double lmax = jacobi_absmax(C_(ux[k]), C_(uy[k]) + 0.5 * o3, C_(wx[k]) + 0.5 * o2, C_(vy[k]),
+                                                  C_(wy[k]) - 0.5 * o1, -(C_(ux[k]) + C_(vy[k])));
+                gg = fmax(gg, lmax);
+                if (z == nz) usgg = fmax(usgg, lmax);
+                if (z == 0) lsgg = fmax(lsgg, lmax);
+            }
+        // characteristic vorticity <|w|^2>/<|w|> over cells with |w| > w_rms
+        // (field_diagnostics.f90:501-545), from the z-midpoint vorticity
+        field_phys(t, a, a.in[9], A, B, xi, false);
+        field_phys(t, a, a.in[10], A, B, et, false);
+        field_phys(t, a, a.in[11], A, B, ze, false);
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k)
+            for (int h = 0; h < 2; ++h) {
+                if (!reg_valid(t, k, h, nz)) continue;
+                if (reg_z(t, k, h) < 1) continue;
+                const double t1 = fabs(C_(xi[k])), t2 = fabs(C_(et[k])), t3 = fabs(C_(ze[k]));
+                if (t1 + t2 + t3 > vortrms) {
+                    l1 += t1 + t2 + t3;
+                    l2 += t1 * t1 + t2 * t2 + t3 * t3;
+                }
+            }
+#undef C_
+        w2max = block_max(w2max, scratch);
+        gg = block_max(gg, scratch);
+        usgg = block_max(usgg, scratch);
+        lsgg = block_max(lsgg, scratch);
+        umax = block_max(umax, scratch);
+        vmax = block_max(vmax, scratch);
+        wmax = block_max(wmax, scratch);
+        l1 = block_sum(l1, scratch);
+        l2 = block_sum(l2, scratch);
+        if (threadIdx.x == 0) {
+            double *p = a.part + (size_t)blockIdx.x * RX_N;
+            p[RX_W2MAX] = w2max;
+            p[RX_GG] = gg;
+            p[RX_USGG] = usgg;
+            p[RX_LSGG] = lsgg;
+            p[RX_UMAX] = umax;
+            p[RX_VMAX] = vmax;
+            p[RX_WMAX] = wmax;
+            p[RX_L1] = l1;
+            p[RX_L2] = l2;
+        }
+        return;
+    }
+}
+
+size_t line_lds_bytes(int n, int ZC) {
+    const size_t tb = (size_t)(ZC / 2) * (n + 1) * sizeof(double2);
+    size_t need = 2 * tb;
+    const size_t red = kLineThreads * sizeof(double);
+    return need > red ? need : red;
+}
+
+hipError_t configure_linepass() {
+    return hipFuncSetAttribute((const void *)k_line, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
+    (void)nthreads;
+    const int blocks = a.nother * a.nzt;
+    hipLaunchKernelGGL(k_line, dim3(blocks), dim3(kLineThreads), line_lds_bytes(a.n, a.ZC), stream, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// finishing reductions (one block)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fin_y(FinArgs a) {
+    __shared__ double red[256];
+    double s[RY_N];
+    for (int q = 0; q < RY_N; ++q) s[q] = 0.0;
+    for (int b = threadIdx.x; b < a.nby; b += blockDim.x)
+        for (int q = 0; q < RY_N; ++q) s[q] += a.party[(size_t)b * RY_N + q];
+    double tot[RY_N];
+    for (int q = 0; q < RY_N; ++q) tot[q] = block_sum(s[q], red);
+    if (threadIdx.x == 0) {
+        // get_kinetic_energy / get_enstrophy: (1/2 interior + 1/4 lids) / ncell
+        // (field_diagnostics.f90:85-206); vortrms = sqrt(get_mean(|w|^2)) (advance.f90:177)
+        a.scal[SC_KE] = 0.5 * tot[RY_UU] * a.ncelli;
+        a.scal[SC_EN] = 0.5 * tot[RY_WW] * a.ncelli;
+        a.scal[SC_HEL] = tot[RY_UW] * a.ncelli;
+        a.scal[SC_VORTRMS] = sqrt(tot[RY_WW] * a.ncelli);
+        a.scal[SC_VORMX] = tot[RY_MX] * a.ncelli;
+        a.scal[SC_VORMY] = tot[RY_MY] * a.ncelli;
+        a.scal[SC_VORMZ] = tot[RY_MZ] * a.ncelli;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fin_x(FinArgs a) {
+    __shared__ double red[256];
+    double w2 = 0.0, gg = 2.220446049250313e-16, us = 0.0, ls = 0.0;
+    double um = -INFINITY, vm = -INFINITY, wm = -INFINITY, l1 = 0.0, l2 = 0.0;
+    for (int b = threadIdx.x; b < a.nbx; b += blockDim.x) {
+        const double *p = a.partx + (size_t)b * RX_N;
+        w2 = fmax(w2, p[RX_W2MAX]);
+        gg = fmax(gg, p[RX_GG]);
+        us = fmax(us, p[RX_USGG]);
+        ls = fmax(ls, p[RX_LSGG]);
+        um = fmax(um, p[RX_UMAX]);
+        vm = fmax(vm, p[RX_VMAX]);
+        wm = fmax(wm, p[RX_WMAX]);
+        l1 += p[RX_L1];
+        l2 += p[RX_L2];
+    }
+    w2 = block_max(w2, red);
+    gg = block_max(gg, red);
+    us = block_max(us, red);
+    ls = block_max(ls, red);
+    um = block_max(um, red);
+    vm = block_max(vm, red);
+    wm = block_max(wm, red);
+    l1 = block_sum(l1, red);
+    l2 = block_sum(l2, red);
+    if (threadIdx.x != 0) return;
+    double *sc = a.scal;
+    const double small = 1.0e-12, cflmax = 0.8;
+    const double t = sc[SC_T];
+    const double vortmax = sqrt(w2);
+    const double vorch = l2 / (small + l1);            // vorl1 starts at small
+    const double bfmax = 0.0;
+    // CFL and strain limits (advance.f90:325-333)
+    const double dtcfl = cflmax * fmin(fmin(a.dx / (um + small), a.dy / (vm + small)), a.dz / (wm + small));
+    double dt = fmin(fmin(a.alpha / (gg + small), a.alpha / (bfmax + small)), fmin(dtcfl, a.limit - t));
+    if (a.dt_fixed > 0.0) dt = fmin(a.dt_fixed, a.limit - t);
+    // rolling mean (rolling_mean.f90:36-69); indices are 1-based as there
+    double sma = sc[SC_RM_SMA];
+    int inew = (int)sc[SC_RM_INEW], iold = (int)sc[SC_RM_IOLD], filled = (int)sc[SC_RM_FILLED];
+    const int len = a.roll_len;
+    if (filled) {
+        const double vold = a.roll_hist[iold - 1];
+        iold = (iold % len) + 1;
+        sma = sma + (gg - vold) / (double)len;
+        a.roll_hist[inew - 1] = gg;
+        inew = (inew % len) + 1;
+    } else {
+        a.roll_hist[inew - 1] = gg;
+        double s = 0.0;
+        for (int i = 0; i < inew; ++i) s += a.roll_hist[i];
+        sma = s / (double)inew;
+        filled = (len == inew);
+        inew = (inew % len) + 1;
+    }
+    sc[SC_RM_SMA] = sma;
+    sc[SC_RM_INEW] = inew;
+    sc[SC_RM_IOLD] = iold;
+    sc[SC_RM_FILLED] = filled;
+    // get_diffusion_pre_factor (advance.f90:381-410)
+    double pref;
+    switch (a.pretype) {
+        case 0: pref = 1.0; break;
+        case 1: pref = vorch; break;
+        case 2: pref = bfmax; break;
+        case 3: pref = sma; break;
+        case 4: pref = gg; break;
+        default: pref = us; break;
+    }
+    sc[SC_PREF] = pref;
+    sc[SC_DT] = dt;
+    if (a.stepper == 1)
+        sc[SC_DFAC] = (a.nnu == 1) ? dt : pref * dt;    // cn2.f90:52-58
+    else
+        sc[SC_DFAC] = 0.5 * pref * dt;                   // impl_rk4.f90:43
+    if (a.diag) {
+        const int row = (int)sc[SC_STEP];
+        if (row < a.max_rows) {
+            double *d = a.diag + (size_t)row * 19;
+            d[0] = t; d[1] = dt; d[2] = sc[SC_KE]; d[3] = sc[SC_EN]; d[4] = sc[SC_HEL];
+            d[5] = vortmax; d[6] = sc[SC_VORTRMS]; d[7] = vorch;
+            d[8] = sc[SC_VORMX]; d[9] = sc[SC_VORMY]; d[10] = sc[SC_VORMZ];
+            d[11] = gg; d[12] = um; d[13] = vm; d[14] = wm; d[15] = us; d[16] = ls; d[17] = sma; d[18] = pref;
+        }
+        sc[SC_STEP] = row + 1;
+    }
+}
+
+hipError_t launch_fin_y(const FinArgs &a, hipStream_t stream) {
+    hipLaunchKernelGGL(k_fin_y, dim3(1), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream) {
+    hipLaunchKernelGGL(k_fin_x, dim3(1), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace ps3d

@@ -1,0 +1,922 @@
+// ps3d_gpu.hip -- C ABI (include/ps3d_gpu.h) and step orchestration.
+//
+// One handle = one grid resident in the HBM of the current device, one HIP stream.
+// The fused RK4 stage is five passes over HBM (z-inverse, y-inverse, x, y-forward,
+// z-forward+update); the adapt stage adds a y-inverse with derivatives, an x pass
+// for the strain/CFL/vorticity diagnostics and two one-block reductions.  dt and
+// the diffusion factor stay on the device, so advance_n queues whole steps without
+// host round trips.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ps3d_gpu.h"
+#include "kernels.hpp"
+
+using namespace ps3d;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(PS3D_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+namespace ps3d {
+hipError_t configure_zpass();
+hipError_t configure_linepass();
+}
+
+struct LineCfg {
+    int n, ZC, nzt, nother;
+    long estride, ostride;
+};
+
+struct ps3d_handle {
+    ps3d_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Grid g{};
+    size_t nvol = 0;               // doubles per padded volume
+    double *pool = nullptr;
+    double *svor[3]{}, *qdi[3]{}, *qdf[3]{}, *svel[3]{}, *vel[3]{}, *vor[3]{}, *grad[3]{}, *mid[3]{};
+    double *scal = nullptr, *party = nullptr, *partx = nullptr, *roll = nullptr;
+    double *diag = nullptr;
+    int diag_cap = 0;
+    double *rkx = nullptr, *rky = nullptr, *filx = nullptr, *fily = nullptr, *filz = nullptr;
+    double2 *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    FftPlan px{}, py{}, pz{};
+    ZLaunch zl{};
+    int TQ = 0, QPB = 0, nquads = 0;
+    LineCfg lx{}, ly{};
+    double visc = 0.0;
+    bool have_state = false, have_phys = false, have_adapt = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int last_steps = 0;
+    std::vector<double> h_rkx, h_rky;
+};
+
+// ---------------------------------------------------------------------------
+// host tables (the reference's formulas, evaluated in the same order)
+// ---------------------------------------------------------------------------
+static double ipow(double x, int p) {
+    double r = 1.0, b = x;
+    int e = p;
+    while (e) {
+        if (e & 1) r *= b;
+        b *= b;
+        e >>= 1;
+    }
+    return r;
+}
+
+// rkx(k) = hrkx(2 min(k, n-k)), hrkx(j) = (pi/L) j  (sta2dfft.f90:70-75, sta3dfft.f90:90-102)
+static std::vector<double> wavenumbers(int n, double L) {
+    const double pi = 3.141592653589793238462643383279502884;
+    const double sc = pi / L;
+    std::vector<double> h(n + 1), rk(n, 0.0);
+    for (int k = 1; k <= n; ++k) h[k] = sc * (double)k;
+    const int nw = n / 2;
+    rk[0] = 0.0;
+    for (int k = 1; k <= nw - 1; ++k) {
+        rk[k] = h[2 * k];
+        rk[n - k] = h[2 * k];
+    }
+    rk[nw] = h[n];
+    return rk;
+}
+
+static std::vector<double2> twiddles(int n) {
+    std::vector<double2> t(n);
+    for (int m = 0; m < n; ++m) {
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        t[m] = make_double2((double)cosl(a), -(double)sinl(a));
+    }
+    return t;
+}
+
+static bool factor23(int n) {
+    if (n < 2) return false;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+}
+
+static LineCfg line_cfg(int n, int nz, int zs, int nother, long estride, long ostride) {
+    LineCfg c;
+    c.n = n;
+    int zc = 4096 / n;
+    if (zc > 16) zc = 16;
+    if (zc < 2) zc = 2;
+    zc &= ~1;
+    c.ZC = zc;
+    c.nzt = (nz + 1 + zc - 1) / zc;
+    c.nother = nother;
+    c.estride = estride;
+    c.ostride = ostride;
+    (void)zs;
+    return c;
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &v) {
+    HIPCHK(hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(1, v.size())));
+    HIPCHK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static ZTabs ztabs(const ps3d_handle *h) {
+    ZTabs t;
+    t.rkx = h->rkx;
+    t.rky = h->rky;
+    t.filx = h->filx;
+    t.fily = h->fily;
+    t.filz = h->filz;
+    t.hou_li = h->p.filtering == PS3D_FILTER_HOU_LI;
+    t.nnu = h->p.nnu;
+    return t;
+}
+
+static void free_handle(ps3d_handle *h) {
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    hipFree(h->pool);
+    hipFree(h->scal);
+    hipFree(h->party);
+    hipFree(h->partx);
+    hipFree(h->roll);
+    hipFree(h->diag);
+    hipFree(h->rkx);
+    hipFree(h->rky);
+    hipFree(h->filx);
+    hipFree(h->fily);
+    hipFree(h->filz);
+    hipFree(h->twx);
+    hipFree(h->twy);
+    hipFree(h->twz);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+// Builds tables, plans and launch shapes. alloc_fields = false for the 1-D helpers.
+static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, bool strict) {
+    h->p = *p;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    if (strict) {
+        if (nx < 2 || ny < 2 || nz < 2 || (nx & 1) || (ny & 1) || (nz & 1))
+            return fail(PS3D_E_ARG, "nx, ny, nz must be even and >= 2");
+        if (!factor23(nx) || !factor23(ny) || !factor23(nz))
+            return fail(PS3D_E_FACTOR, "Factorisation not possible: grid lengths must be 2^a 3^b");
+        if (nx > 4096 || ny > 4096 || nz > 1024) return fail(PS3D_E_UNSUPPORTED, "grid too large (nx,ny <= 4096, nz <= 1024)");
+        for (int d = 0; d < 3; ++d)
+            if (!(p->extent[d] > 0.0)) return fail(PS3D_E_ARG, "Domain length equal to zero not allowed");
+        if (p->nnu < 1) return fail(PS3D_E_ARG, "nnu must be >= 1");
+        if (p->filtering != PS3D_FILTER_HOU_LI && p->filtering != PS3D_FILTER_23RULE)
+            return fail(PS3D_E_ARG, "filtering must be 'Hou & Li' or '2/3-rule'");
+        if (p->stepper != PS3D_STEPPER_IMPL_RK4 && p->stepper != PS3D_STEPPER_CN2)
+            return fail(PS3D_E_ARG, "unknown stepper");
+        if (p->pretype < 0 || p->pretype > 5) return fail(PS3D_E_ARG, "unknown pretype");
+        if (p->length_scale != PS3D_LS_KOLMOGOROV && p->length_scale != PS3D_LS_GEOPHYSICAL)
+            return fail(PS3D_E_ARG, "We only support 'Kolmogorov' or 'geophysical'");
+        if (p->roll_mean_win_size < 1) return fail(PS3D_E_ARG, "roll_mean_win_size must be >= 1");
+    }
+    HIPCHK(hipGetDevice(&h->device));
+    HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&h->ev0));
+    HIPCHK(hipEventCreate(&h->ev1));
+    Grid &g = h->g;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.zs = ((nz + 1 + kZAlign - 1) / kZAlign) * kZAlign;
+    g.nwx = nx / 2;
+    g.nwy = ny / 2;
+    g.dz = p->extent[2] / (double)nz;
+    g.dzi = 1.0 / g.dz;
+    g.hdzi = 0.5 * g.dzi;
+    g.lz = p->extent[2];
+    g.zlo = p->lower[2];
+    h->nvol = (size_t)nx * ny * g.zs;
+
+    h->h_rkx = wavenumbers(nx, p->extent[0]);
+    h->h_rky = wavenumbers(ny, p->extent[1]);
+    std::vector<double> rkz(nz + 1, 0.0);
+    for (int k = 1; k <= nz; ++k) rkz[k] = (3.141592653589793238462643383279502884 / p->extent[2]) * (double)k;
+    // filters (inversion_utils.f90:377-455)
+    auto maxv = [](const std::vector<double> &v) { double m = 0; for (double x : v) m = std::max(m, x); return m; };
+    std::vector<double> fx(nx), fy(ny), fz(nz + 1);
+    const double mx = maxv(h->h_rkx), my = maxv(h->h_rky), mz = maxv(rkz);
+    if (p->filtering == PS3D_FILTER_HOU_LI) {
+        const double ix = 1.0 / mx, iy = 1.0 / my, iz = 1.0 / mz;
+        for (int k = 0; k < nx; ++k) fx[k] = -36.0 * ipow(ix * h->h_rkx[k], 36);
+        for (int k = 0; k < ny; ++k) fy[k] = -36.0 * ipow(iy * h->h_rky[k], 36);
+        for (int k = 0; k <= nz; ++k) fz[k] = -36.0 * ipow(iz * rkz[k], 36);
+    } else {
+        const double f23 = 2.0 / 3.0;
+        for (int k = 0; k < nx; ++k) fx[k] = (h->h_rkx[k] <= f23 * mx) ? 1.0 : 0.0;
+        for (int k = 0; k < ny; ++k) fy[k] = (h->h_rky[k] <= f23 * my) ? 1.0 : 0.0;
+        for (int k = 0; k <= nz; ++k) fz[k] = (rkz[k] <= f23 * mz) ? 1.0 : 0.0;
+    }
+    int st;
+    if ((st = upload(&h->rkx, h->h_rkx)) || (st = upload(&h->rky, h->h_rky)) || (st = upload(&h->filx, fx)) ||
+        (st = upload(&h->fily, fy)) || (st = upload(&h->filz, fz)))
+        return st;
+    if (!make_plan(nx, h->px) || !make_plan(ny, h->py) || !make_plan(2 * nz, h->pz))
+        return fail(PS3D_E_FACTOR, "Factorisation not possible: grid lengths must be 2^a 3^b");
+    if ((st = upload(&h->twx, twiddles(nx))) || (st = upload(&h->twy, twiddles(ny))) ||
+        (st = upload(&h->twz, twiddles(2 * nz))))
+        return st;
+
+    // z launch shape: TQ threads per quad, ZPT levels per thread, 256 threads per block
+    const int nz1 = nz + 1;
+    if (nz1 <= 64) { h->TQ = 64; h->zl.zpt = 1; }
+    else if (nz1 <= 128) { h->TQ = 128; h->zl.zpt = 1; }
+    else if (nz1 <= 256) { h->TQ = 256; h->zl.zpt = 1; }
+    else if (nz1 <= 512) { h->TQ = 256; h->zl.zpt = 2; }
+    else if (nz1 <= 768) { h->TQ = 256; h->zl.zpt = 3; }
+    else { h->TQ = 256; h->zl.zpt = 5; }
+    h->QPB = 256 / h->TQ;
+    h->nquads = (nx / 2 + 1) * (ny / 2 + 1);
+    h->zl.blocks = (h->nquads + h->QPB - 1) / h->QPB;
+    h->zl.lds = zpass_lds_bytes(h->QPB, nz);
+    if (h->zl.lds > 160 * 1024) return fail(PS3D_E_UNSUPPORTED, "nz too large for the LDS z transform");
+
+    h->ly = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs);
+    h->lx = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs);
+
+    HIPCHK(ps3d::configure_zpass());
+    HIPCHK(ps3d::configure_linepass());
+
+    HIPCHK(hipMalloc((void **)&h->scal, sizeof(double) * SC_NSCAL));
+    HIPCHK(hipMemset(h->scal, 0, sizeof(double) * SC_NSCAL));
+    HIPCHK(hipMalloc((void **)&h->party, sizeof(double) * (size_t)h->ly.nother * h->ly.nzt * RY_N));
+    HIPCHK(hipMalloc((void **)&h->partx, sizeof(double) * (size_t)h->lx.nother * h->lx.nzt * RX_N));
+    HIPCHK(hipMalloc((void **)&h->roll, sizeof(double) * std::max(1, p->roll_mean_win_size)));
+    if (alloc_fields) {
+        const size_t nv = 24;
+        const size_t bytes = sizeof(double) * h->nvol * nv;
+        hipError_t e = hipMalloc((void **)&h->pool, bytes);
+        if (e != hipSuccess) return fail(PS3D_E_NOMEM, "device allocation of the field pool failed");
+        HIPCHK(hipMemset(h->pool, 0, bytes));
+        double *q = h->pool;
+        auto take3 = [&](double **d) { for (int c = 0; c < 3; ++c) { d[c] = q; q += h->nvol; } };
+        take3(h->svor); take3(h->qdi); take3(h->qdf); take3(h->svel);
+        take3(h->vel); take3(h->vor); take3(h->grad); take3(h->mid);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+static LineArgs line_args(const ps3d_handle *h, int axis, int mode) {
+    LineArgs a;
+    std::memset(&a, 0, sizeof a);
+    const LineCfg &c = axis == 0 ? h->lx : h->ly;
+    a.g = h->g;
+    a.axis = axis;
+    a.n = c.n;
+    a.estride = c.estride;
+    a.ostride = c.ostride;
+    a.nother = c.nother;
+    a.ZC = c.ZC;
+    a.nzt = c.nzt;
+    a.P = axis == 0 ? h->px : h->py;
+    a.tw = axis == 0 ? h->twx : h->twy;
+    a.rk = axis == 0 ? h->rkx : h->rky;
+    a.nwl = c.n / 2;
+    a.mode = mode;
+    a.scal = h->scal;
+    for (int d = 0; d < 3; ++d) a.fcor[d] = h->p.f_cor[d];
+    a.ncelli = 1.0 / ((double)h->g.nx * h->g.ny * h->g.nz);
+    a.sqrt_nx = std::sqrt((double)h->g.nx);
+    return a;
+}
+
+static int run_line(ps3d_handle *h, const LineArgs &a) {
+    HIPCHK(launch_line(a, 0, h->stream));
+    return 0;
+}
+
+static int line_fields(ps3d_handle *h, int axis, int mode, int nf, double *const *in, double *const *out) {
+    LineArgs a = line_args(h, axis, mode);
+    a.nf = nf;
+    for (int f = 0; f < nf; ++f) {
+        a.in[f] = in[f];
+        a.out[f] = out[f];
+    }
+    return run_line(h, a);
+}
+
+static FinArgs fin_args(ps3d_handle *h) {
+    FinArgs f;
+    std::memset(&f, 0, sizeof f);
+    f.party = h->party;
+    f.nby = h->ly.nother * h->ly.nzt;
+    f.partx = h->partx;
+    f.nbx = h->lx.nother * h->lx.nzt;
+    f.scal = h->scal;
+    f.diag = h->diag;
+    f.max_rows = h->diag_cap;
+    f.ncelli = 1.0 / ((double)h->g.nx * h->g.ny * h->g.nz);
+    f.alpha = h->p.alpha;
+    f.limit = h->p.limit;
+    f.dt_fixed = h->p.dt_fixed;
+    f.dx = h->p.extent[0] / h->g.nx;
+    f.dy = h->p.extent[1] / h->g.ny;
+    f.dz = h->p.extent[2] / h->g.nz;
+    f.pretype = h->p.pretype;
+    f.stepper = h->p.stepper;
+    f.nnu = h->p.nnu;
+    f.roll_len = h->p.roll_mean_win_size;
+    f.roll_hist = h->roll;
+    return f;
+}
+
+static ZInvArgs zinv_args(ps3d_handle *h) {
+    ZInvArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.g = h->g;
+    a.tb = ztabs(h);
+    a.TQ = h->TQ;
+    a.QPB = h->QPB;
+    a.nquads = h->nquads;
+    a.pz = h->pz;
+    a.twz = h->twz;
+    a.zscale = 1.0 / std::sqrt(2.0 * h->g.nz);
+    a.rkz_sc = 3.141592653589793238462643383279502884 / h->p.extent[2];
+    for (int c = 0; c < 3; ++c) a.svor[c] = h->svor[c];
+    for (int c = 0; c < 3; ++c) {
+        a.out[c] = h->svel[c];
+        a.out[3 + c] = h->vor[c];
+    }
+    return a;
+}
+
+static ZTendArgs ztend_args(ps3d_handle *h, int mode, int stage, double *const *rqp) {
+    ZTendArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.g = h->g;
+    a.tb = ztabs(h);
+    a.TQ = h->TQ;
+    a.QPB = h->QPB;
+    a.nquads = h->nquads;
+    a.pz = h->pz;
+    a.twz = h->twz;
+    a.zscale = 1.0 / std::sqrt(2.0 * h->g.nz);
+    for (int c = 0; c < 3; ++c) {
+        a.rqp[c] = rqp ? rqp[c] : nullptr;
+        a.svorts[c] = h->mid[c];
+        a.rk.q[c] = h->svor[c];
+        a.rk.qdi[c] = h->qdi[c];
+        a.rk.qdf[c] = h->qdf[c];
+    }
+    a.rk.scal = h->scal;
+    a.mode = mode;
+    a.stage = stage;
+    return a;
+}
+
+static ZOpArgs zop_args(ps3d_handle *h, int op, const double *in, double *out) {
+    ZOpArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.g = h->g;
+    a.tb = ztabs(h);
+    a.TQ = h->TQ;
+    a.QPB = h->QPB;
+    a.nquads = h->nquads;
+    a.pz = h->pz;
+    a.twz = h->twz;
+    a.zscale = 1.0 / std::sqrt(2.0 * h->g.nz);
+    a.in = in;
+    a.out = out;
+    a.op = op;
+    return a;
+}
+
+static int run_mean(ps3d_handle *h, int mode, int advance_time) {
+    MeanArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.g = h->g;
+    a.pz = h->pz;
+    a.twz = h->twz;
+    a.zscale = 1.0 / std::sqrt(2.0 * h->g.nz);
+    a.svor[0] = h->svor[0];
+    a.svor[1] = h->svor[1];
+    a.scal = h->scal;
+    a.mode = mode;
+    a.advance_time = advance_time;
+    HIPCHK(launch_mean(a, h->g.nz, h->stream));
+    return 0;
+}
+
+// vor2vel + the adapt-stage y pass: leaves u, v, w, xi, eta, zeta, their y
+// derivatives and z midpoints x-spectral / y-physical, svel semi-spectral, the
+// projected svor (project = true) and the Parseval sums finished into scal.
+static int inverse_adapt_stage(ps3d_handle *h, bool project) {
+    ZInvArgs z = zinv_args(h);
+    if (project) {
+        z.svor_out[0] = h->svor[0];
+        z.svor_out[1] = h->svor[1];
+    }
+    for (int c = 0; c < 3; ++c) z.mid[c] = h->mid[c];
+    HIPCHK(launch_zinv(h->zl, z, h->stream));
+    LineArgs y = line_args(h, 1, LM_INV_Y1);
+    for (int c = 0; c < 3; ++c) {
+        y.in[c] = h->svel[c];
+        y.out[c] = h->vel[c];
+        y.in[3 + c] = h->vor[c];
+        y.out[3 + c] = h->vor[c];
+        y.in[6 + c] = h->mid[c];
+        y.out[6 + c] = h->mid[c];
+        y.out[9 + c] = h->grad[c];
+    }
+    y.part = h->party;
+    int st = run_line(h, y);
+    if (st) return st;
+    HIPCHK(launch_fin_y(fin_args(h), h->stream));
+    return 0;
+}
+
+static int x_adapt(ps3d_handle *h, int phys_in) {
+    LineArgs x = line_args(h, 0, LM_XADAPT);
+    x.phys_in = phys_in;
+    for (int c = 0; c < 3; ++c) {
+        x.in[c] = h->vel[c];
+        x.in[3 + c] = h->vor[c];
+        x.in[6 + c] = h->grad[c];
+        x.in[9 + c] = h->mid[c];
+    }
+    x.part = h->partx;
+    int st = run_line(h, x);
+    if (st) return st;
+    HIPCHK(launch_fin_x(fin_args(h), h->stream));
+    return 0;
+}
+
+// x pass (u x omega) + y forward: r, q, p left semi-spectral in dst[]
+static int products(ps3d_handle *h, int phys_in, double *const *dst) {
+    LineArgs x = line_args(h, 0, LM_XMAIN);
+    x.phys_in = phys_in;
+    for (int c = 0; c < 3; ++c) {
+        x.in[c] = h->vel[c];
+        x.in[3 + c] = h->vor[c];
+        x.out[c] = dst[c];
+    }
+    int st = run_line(h, x);
+    if (st) return st;
+    return line_fields(h, 1, LM_FWD, 3, dst, dst);
+}
+
+// One fused RK stage (impl_rk4) or cn2 iteration.  stage 1 includes adapt.
+static int fused_stage(ps3d_handle *h, int mode, int stage) {
+    int st;
+    if (stage == 1) {
+        if ((st = inverse_adapt_stage(h, true))) return st;
+        if ((st = x_adapt(h, 0))) return st;
+    } else {
+        ZInvArgs z = zinv_args(h);
+        HIPCHK(launch_zinv(h->zl, z, h->stream));
+        double *in[6] = {h->svel[0], h->svel[1], h->svel[2], h->vor[0], h->vor[1], h->vor[2]};
+        double *out[6] = {h->vel[0], h->vel[1], h->vel[2], h->vor[0], h->vor[1], h->vor[2]};
+        if ((st = line_fields(h, 1, LM_INV, 6, in, out))) return st;
+    }
+    if ((st = products(h, 0, h->vel))) return st;
+    ZTendArgs t = ztend_args(h, mode, stage, h->vel);
+    HIPCHK(launch_ztend(h->zl, t, h->stream));
+    return 0;
+}
+
+static int one_step(ps3d_handle *h) {
+    int st;
+    if (h->p.stepper == PS3D_STEPPER_CN2) {
+        for (int it = 0; it < 3; ++it) {
+            if ((st = fused_stage(h, ZT_CN2, it == 0 ? 1 : 2))) return st;
+            if ((st = run_mean(h, MEAN_ADJUST, it == 2 ? 2 : 0))) return st;
+        }
+        return 0;
+    }
+    for (int s = 1; s <= 4; ++s)
+        if ((st = fused_stage(h, ZT_RK4, s))) return st;
+    return run_mean(h, MEAN_ADJUST, 1);
+}
+
+// host <-> device copies between ps3d's (0:nz, y, x) layout and the padded columns
+static int copy_in(ps3d_handle *h, double *dst, const double *src) {
+    const size_t w = sizeof(double) * (h->g.nz + 1);
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(double) * h->g.zs, src, w, w, (size_t)h->g.nx * h->g.ny, hipMemcpyHostToDevice,
+                            h->stream));
+    return 0;
+}
+static int copy_out(ps3d_handle *h, double *dst, const double *src) {
+    const size_t w = sizeof(double) * (h->g.nz + 1);
+    HIPCHK(hipMemcpy2DAsync(dst, w, src, sizeof(double) * h->g.zs, w, (size_t)h->g.nx * h->g.ny, hipMemcpyDeviceToHost,
+                            h->stream));
+    return 0;
+}
+
+static int read_scal(ps3d_handle *h, double *out) {
+    HIPCHK(hipMemcpyAsync(out, h->scal, sizeof(double) * SC_NSCAL, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+static int write_scal(ps3d_handle *h, int idx, double v) {
+    HIPCHK(hipMemcpyAsync(h->scal + idx, &v, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static int ensure_diag(ps3d_handle *h, int rows) {
+    if (rows <= h->diag_cap) return 0;
+    hipFree(h->diag);
+    h->diag = nullptr;
+    HIPCHK(hipMalloc((void **)&h->diag, sizeof(double) * PS3D_NDIAG * (size_t)rows));
+    h->diag_cap = rows;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *ps3d_gpu_last_error(void) { return g_err.c_str(); }
+const char *ps3d_gpu_version(void) { return "ps3d-mi355x 0.1 (gfx950, FP64)"; }
+
+void ps3d_params_default(ps3d_params *p) {
+    std::memset(p, 0, sizeof *p);
+    p->nx = p->ny = p->nz = 32;
+    for (int d = 0; d < 3; ++d) p->extent[d] = 1.0;
+    p->filtering = PS3D_FILTER_HOU_LI;
+    p->stepper = PS3D_STEPPER_IMPL_RK4;
+    p->nnu = 3;
+    p->prediss = 10.0;
+    p->pretype = PS3D_PRE_ROLL_MEAN_MAX_STRAIN;
+    p->roll_mean_win_size = 1000;
+    p->length_scale = PS3D_LS_KOLMOGOROV;
+    p->alpha = 0.1;
+    p->limit = 0.0;
+    p->dt_fixed = 0.0;
+}
+
+int ps3d_gpu_create(const ps3d_params *p, ps3d_handle **out) {
+    if (!p || !out) return fail(PS3D_E_ARG, "null argument");
+    *out = nullptr;
+    ps3d_handle *h = new ps3d_handle();
+    int st = init_handle(h, p, true, true);
+    if (st) {
+        std::string msg = g_err;
+        free_handle(h);
+        g_err = msg;
+        return st;
+    }
+    *out = h;
+    return 0;
+}
+
+int ps3d_gpu_destroy(ps3d_handle *h) {
+    free_handle(h);
+    return 0;
+}
+
+int ps3d_gpu_sync(ps3d_handle *h) {
+    if (!h) return fail(PS3D_E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// setup_fields (utils.f90:136-184)
+int ps3d_gpu_set_vorticity(ps3d_handle *h, const double *vor) {
+    if (!h || !vor) return fail(PS3D_E_ARG, "null argument");
+    const size_t hv = (size_t)h->g.nx * h->g.ny * (h->g.nz + 1);
+    int st;
+    for (int c = 0; c < 3; ++c)
+        if ((st = copy_in(h, h->vel[c], vor + c * hv))) return st;
+    // field_decompose_physical: fftxyp2s (y then x) + decompose
+    if ((st = line_fields(h, 1, LM_FWD, 3, h->vel, h->vel))) return st;
+    if ((st = line_fields(h, 0, LM_FWD, 3, h->vel, h->vel))) return st;
+    for (int c = 0; c < 3; ++c) HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DECOMPOSE, h->vel[c], h->svor[c]), h->stream));
+    if ((st = run_mean(h, MEAN_CALC, 0))) return st;
+    HIPCHK(hipMemsetAsync(h->scal + SC_T, 0, sizeof(double), h->stream));
+    double init[4] = {0.0, 1.0, 1.0, 0.0};   // rolling mean: sma, inew, iold, filled
+    HIPCHK(hipMemcpyAsync(h->scal + SC_RM_SMA, init, sizeof init, hipMemcpyHostToDevice, h->stream));
+    h->have_state = true;
+    if ((st = ps3d_gpu_vor2vel(h))) return st;
+    double sc[SC_NSCAL];
+    if ((st = read_scal(h, sc))) return st;
+    // init_diffusion -> get_viscosity (inversion_utils.f90:124-184)
+    const double te = sc[SC_KE], en = sc[SC_EN];
+    double rkxmax = 0, rkymax = 0;
+    for (double v : h->h_rkx) rkxmax = std::max(rkxmax, v);
+    for (double v : h->h_rky) rkymax = std::max(rkymax, v);
+    const double K2max = std::pow(std::max(rkxmax, rkymax), 2);
+    const double rkmsi = 1.0 / K2max;
+    if (h->p.length_scale == PS3D_LS_KOLMOGOROV)
+        h->visc = h->p.prediss * std::pow(K2max * te / en, 1.0 / 3.0) * ipow(rkmsi, h->p.nnu);
+    else
+        h->visc = h->p.prediss * ipow(rkmsi, h->p.nnu);
+    return write_scal(h, SC_VISC, h->visc);
+}
+
+// vor2vel: device state -> physical vel, vor (and y-derivatives / midpoints for adapt)
+int ps3d_gpu_vor2vel(ps3d_handle *h) {
+    if (!h) return fail(PS3D_E_ARG, "null handle");
+    if (!h->have_state) return fail(PS3D_E_STATE, "vor2vel before set_vorticity");
+    int st;
+    if ((st = inverse_adapt_stage(h, true))) return st;
+    double *f[12];
+    for (int c = 0; c < 3; ++c) {
+        f[c] = h->vel[c];
+        f[3 + c] = h->vor[c];
+        f[6 + c] = h->grad[c];
+        f[9 + c] = h->mid[c];
+    }
+    if ((st = line_fields(h, 0, LM_INV, 12, f, f))) return st;
+    h->have_phys = true;
+    h->have_adapt = false;
+    return 0;
+}
+
+int ps3d_gpu_adapt(ps3d_handle *h, double t, double *dt, double *diag) {
+    if (!h) return fail(PS3D_E_ARG, "null handle");
+    if (!h->have_phys) return fail(PS3D_E_STATE, "adapt needs a preceding vor2vel");
+    int st;
+    if ((st = ensure_diag(h, 1))) return st;
+    double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(h->scal + SC_T, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->scal + SC_STEP, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((st = x_adapt(h, 1))) return st;
+    double row[PS3D_NDIAG];
+    HIPCHK(hipMemcpyAsync(row, h->diag, sizeof row, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (dt) *dt = row[PS3D_D_DT];
+    if (diag) std::memcpy(diag, row, sizeof row);
+    h->have_adapt = true;
+    return 0;
+}
+
+int ps3d_gpu_source(ps3d_handle *h) {
+    if (!h) return fail(PS3D_E_ARG, "null handle");
+    if (!h->have_phys) return fail(PS3D_E_STATE, "source needs a preceding vor2vel");
+    int st;
+    // r, q, p into grad (adapt has consumed it), then the z part into svorts (= mid)
+    if ((st = products(h, 1, h->grad))) return st;
+    ZTendArgs t = ztend_args(h, ZT_STORE, 0, h->grad);
+    HIPCHK(launch_ztend(h->zl, t, h->stream));
+    return 0;
+}
+
+int ps3d_gpu_step(ps3d_handle *h, double *t, double dt) {
+    if (!h || !t) return fail(PS3D_E_ARG, "null argument");
+    if (!h->have_adapt) return fail(PS3D_E_STATE, "step needs vor2vel, adapt and source first");
+    int st;
+    HIPCHK(hipMemcpyAsync(h->scal + SC_T, t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->scal + SC_DT, &dt, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    const int mode = h->p.stepper == PS3D_STEPPER_CN2 ? ZT_CN2 : ZT_RK4;
+    ZTendArgs a = ztend_args(h, mode, 1, nullptr);
+    HIPCHK(launch_zrk(h->zl, a, h->stream));
+    if (mode == ZT_CN2) {
+        if ((st = run_mean(h, MEAN_ADJUST, 0))) return st;
+        for (int it = 1; it <= 2; ++it) {
+            if ((st = fused_stage(h, ZT_CN2, 2))) return st;
+            if ((st = run_mean(h, MEAN_ADJUST, it == 2 ? 2 : 0))) return st;
+        }
+    } else {
+        for (int s = 2; s <= 4; ++s)
+            if ((st = fused_stage(h, ZT_RK4, s))) return st;
+        if ((st = run_mean(h, MEAN_ADJUST, 1))) return st;
+    }
+    double sc[SC_NSCAL];
+    if ((st = read_scal(h, sc))) return st;
+    *t = sc[SC_T];
+    h->have_phys = false;
+    h->have_adapt = false;
+    return 0;
+}
+
+int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_series) {
+    if (!h || !t || nsteps < 0) return fail(PS3D_E_ARG, "invalid argument");
+    if (!h->have_state) return fail(PS3D_E_STATE, "advance before set_vorticity");
+    int st;
+    if ((st = ensure_diag(h, std::max(1, nsteps)))) return st;
+    double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(h->scal + SC_T, t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->scal + SC_STEP, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int s = 0; s < nsteps; ++s)
+        if ((st = one_step(h))) return st;
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    h->last_steps = nsteps;
+    double sc[SC_NSCAL];
+    if ((st = read_scal(h, sc))) return st;
+    *t = sc[SC_T];
+    if (diag_series && nsteps > 0)
+        HIPCHK(hipMemcpy(diag_series, h->diag, sizeof(double) * PS3D_NDIAG * (size_t)nsteps, hipMemcpyDeviceToHost));
+    h->have_phys = false;
+    h->have_adapt = false;
+    return 0;
+}
+
+int ps3d_gpu_rk_substep(ps3d_handle *h, int stage, double dt, const double *vdiss) {
+    if (!h || !vdiss || stage < 1 || stage > 4) return fail(PS3D_E_ARG, "invalid argument");
+    const size_t n2 = (size_t)h->g.nx * h->g.ny;
+    double *dv = nullptr;
+    HIPCHK(hipMalloc((void **)&dv, sizeof(double) * n2));
+    HIPCHK(hipMemcpy(dv, vdiss, sizeof(double) * n2, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(h->scal + SC_DT, &dt, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    ZTendArgs a = ztend_args(h, ZT_RK4, stage, nullptr);
+    a.rk.vdiss = dv;
+    hipError_t e = launch_zrk(h->zl, a, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(dv);
+    if (e != hipSuccess) return fail(PS3D_E_HIP, std::string("rk_substep: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int ps3d_gpu_last_timing(ps3d_handle *h, double *ms, int *nsteps) {
+    if (!h) return fail(PS3D_E_ARG, "null handle");
+    if (ms) *ms = h->last_ms;
+    if (nsteps) *nsteps = h->last_steps;
+    return 0;
+}
+
+int ps3d_gpu_derived(ps3d_handle *h) {
+    (void)h;
+    return fail(PS3D_E_UNSUPPORTED, "pressure / horizontal divergence not available yet");
+}
+
+int ps3d_gpu_get(ps3d_handle *h, int which, double *host) {
+    if (!h || !host) return fail(PS3D_E_ARG, "null argument");
+    const size_t hv = (size_t)h->g.nx * h->g.ny * (h->g.nz + 1);
+    double *const *src = nullptr;
+    switch (which) {
+        case PS3D_SVOR: src = h->svor; break;
+        case PS3D_VOR: src = h->vor; break;
+        case PS3D_VEL: src = h->vel; break;
+        case PS3D_SVEL: src = h->svel; break;
+        case PS3D_SVORTS: src = h->mid; break;
+        default: return fail(PS3D_E_UNSUPPORTED, "field not available");
+    }
+    int st;
+    for (int c = 0; c < 3; ++c)
+        if ((st = copy_out(h, host + c * hv, src[c]))) return st;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int ps3d_gpu_set(ps3d_handle *h, int which, const double *host) {
+    if (!h || !host) return fail(PS3D_E_ARG, "null argument");
+    const size_t hv = (size_t)h->g.nx * h->g.ny * (h->g.nz + 1);
+    double *const *dst = nullptr;
+    switch (which) {
+        case PS3D_SVOR: dst = h->svor; h->have_state = true; break;
+        case PS3D_SVORTS: dst = h->mid; break;
+        default: return fail(PS3D_E_UNSUPPORTED, "field cannot be set");
+    }
+    int st;
+    for (int c = 0; c < 3; ++c)
+        if ((st = copy_in(h, dst[c], host + c * hv))) return st;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int ps3d_gpu_op(ps3d_handle *h, int op, const double *in, double *out) {
+    if (!h || !in || !out) return fail(PS3D_E_ARG, "null argument");
+    double *A = h->grad[0], *B = h->grad[1];
+    int st;
+    if ((st = copy_in(h, A, in))) return st;
+    double *a1[1] = {A};
+    double *b1[1] = {B};
+    double *res = B;
+    switch (op) {
+        case PS3D_OP_FFTXYP2S:
+            if ((st = line_fields(h, 1, LM_FWD, 1, a1, b1)) || (st = line_fields(h, 0, LM_FWD, 1, b1, b1))) return st;
+            break;
+        case PS3D_OP_FFTXYS2P:
+            if ((st = line_fields(h, 0, LM_INV, 1, a1, b1)) || (st = line_fields(h, 1, LM_INV, 1, b1, b1))) return st;
+            break;
+        case PS3D_OP_DIFFX:
+            if ((st = line_fields(h, 0, LM_DIFF, 1, a1, b1))) return st;
+            break;
+        case PS3D_OP_DIFFY:
+            if ((st = line_fields(h, 1, LM_DIFF, 1, a1, b1))) return st;
+            break;
+        case PS3D_OP_FFTSINE: HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DST, A, B), h->stream)); break;
+        case PS3D_OP_FFTCOSINE: HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DCT, A, B), h->stream)); break;
+        case PS3D_OP_CENTRAL_DIFFZ: HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DIFFZ, A, B), h->stream)); break;
+        case PS3D_OP_COMBINE_SEMI: HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_COMBINE, A, B), h->stream)); break;
+        case PS3D_OP_DECOMPOSE_SEMI: HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DECOMPOSE, A, B), h->stream)); break;
+        case PS3D_OP_COMBINE_PHYS:
+            HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_COMBINE, A, B), h->stream));
+            if ((st = line_fields(h, 0, LM_INV, 1, b1, b1)) || (st = line_fields(h, 1, LM_INV, 1, b1, b1))) return st;
+            break;
+        case PS3D_OP_DECOMPOSE_PHYS:
+            if ((st = line_fields(h, 1, LM_FWD, 1, a1, a1)) || (st = line_fields(h, 0, LM_FWD, 1, a1, a1))) return st;
+            HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DECOMPOSE, A, B), h->stream));
+            break;
+        default:
+            return fail(PS3D_E_ARG, "unknown op");
+    }
+    if ((st = copy_out(h, out, res))) return st;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// Batched stafft transforms on x(m, n) (m fastest) through the line / z kernels.
+int ps3d_gpu_fft1d(int kind, int m, int n, double *x) {
+    if (!x || m < 1 || n < 2 || (n & 1) || !factor23(n)) return fail(PS3D_E_FACTOR, "fft1d: n must be even and 2^a 3^b");
+    if (kind < 0 || kind > 3) return fail(PS3D_E_ARG, "fft1d: unknown kind");
+    ps3d_params p;
+    ps3d_params_default(&p);
+    ps3d_handle *h = new ps3d_handle();
+    int st;
+    if (kind <= 1) {
+        // lines of length n along "y", m lines along "z": x(m, n) is the column
+        // layout with one x slot, ny = n and nz + 1 = m
+        p.nx = 2; p.ny = n; p.nz = 2;
+        st = init_handle(h, &p, false, false);
+        if (!st) {
+            h->g.nz = m - 1;
+            h->g.zs = ((m + kZAlign - 1) / kZAlign) * kZAlign;
+            h->ly = line_cfg(n, m - 1, h->g.zs, 1, (long)h->g.zs, (long)n * h->g.zs);
+            double *buf = nullptr;
+            if (hipMalloc((void **)&buf, sizeof(double) * (size_t)n * h->g.zs) != hipSuccess) st = fail(PS3D_E_NOMEM, "fft1d alloc");
+            if (!st) {
+                const size_t w = sizeof(double) * m;
+                if (hipMemcpy2D(buf, sizeof(double) * h->g.zs, x, w, w, n, hipMemcpyHostToDevice) != hipSuccess)
+                    st = fail(PS3D_E_HIP, "fft1d copy in");
+                double *a1[1] = {buf};
+                if (!st) st = line_fields(h, 1, kind == 0 ? LM_FWD : LM_INV, 1, a1, a1);
+                if (!st && hipStreamSynchronize(h->stream) != hipSuccess) st = fail(PS3D_E_HIP, "fft1d sync");
+                if (!st && hipMemcpy2D(x, w, buf, sizeof(double) * h->g.zs, w, n, hipMemcpyDeviceToHost) != hipSuccess)
+                    st = fail(PS3D_E_HIP, "fft1d copy out");
+                hipFree(buf);
+            }
+        }
+    } else {
+        // columns of the z kernels: nx = 2, ny = ceil(m/2) (even), nz = n; column c
+        // of x is column c of the device volume
+        const int ncol = m;
+        p.nx = 2;
+        p.ny = 2;
+        while (2 * p.ny < ncol) p.ny *= 2;   // a power of two keeps the unused y plan valid
+        p.nz = n;
+        if (n > 1024) { delete h; return fail(PS3D_E_UNSUPPORTED, "fft1d: dct/dst length <= 1024"); }
+        st = init_handle(h, &p, false, false);
+        if (!st) {
+            const size_t nv = (size_t)p.nx * p.ny * h->g.zs;
+            double *buf = nullptr;
+            if (hipMalloc((void **)&buf, sizeof(double) * 2 * nv) != hipSuccess) st = fail(PS3D_E_NOMEM, "fft1d alloc");
+            if (!st) {
+                hipMemset(buf, 0, sizeof(double) * 2 * nv);
+                std::vector<double> hb((size_t)p.nx * p.ny * h->g.zs, 0.0);
+                for (int c = 0; c < ncol; ++c)
+                    for (int j = 0; j <= n; ++j) {
+                        double v = 0.0;
+                        if (kind == 2) v = x[c + (size_t)m * j];
+                        else if (j >= 1) v = x[c + (size_t)m * (j - 1)];
+                        hb[(size_t)c * h->g.zs + j] = v;
+                    }
+                hipMemcpy(buf, hb.data(), sizeof(double) * nv, hipMemcpyHostToDevice);
+                ZOpArgs za = zop_args(h, kind == 2 ? ZOP_DCT : ZOP_DST, buf, buf + nv);
+                if (launch_zop(h->zl, za, h->stream) != hipSuccess) st = fail(PS3D_E_HIP, "fft1d z launch");
+                if (!st && hipStreamSynchronize(h->stream) != hipSuccess) st = fail(PS3D_E_HIP, "fft1d sync");
+                if (!st) {
+                    hipMemcpy(hb.data(), buf + nv, sizeof(double) * nv, hipMemcpyDeviceToHost);
+                    for (int c = 0; c < ncol; ++c)
+                        for (int j = 0; j <= n; ++j) {
+                            if (kind == 2) x[c + (size_t)m * j] = hb[(size_t)c * h->g.zs + j];
+                            else if (j >= 1) x[c + (size_t)m * (j - 1)] = hb[(size_t)c * h->g.zs + j];
+                        }
+                }
+                hipFree(buf);
+            }
+        }
+    }
+    std::string msg = g_err;
+    free_handle(h);
+    g_err = msg;
+    return st;
+}
+
+}  // extern "C"

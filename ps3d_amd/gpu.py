@@ -1,0 +1,226 @@
+"""Host side of the MI355X pseudo-spectral step: ctypes binding of the C ABI
+(include/ps3d_gpu.h) with the reference's operator names.
+
+The entry points mirror ps3d's module procedures (src/inversion/inversion.f90,
+src/stepper/advance.f90, src/stepper/impl_rk4.f90, src/fft/sta3dfft.f90):
+vor2vel, source, adapt, step, advance, fftxyp2s, diffx, ...  Arrays use ps3d's
+field layout: numpy (nx, ny, nz+1) C order = Fortran (0:nz, 0:ny-1, 0:nx-1),
+vector fields (3, nx, ny, nz+1).  Errors raise Ps3dError carrying the C status
+and message, in place of the reference's mpi_stop.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .params import Params, make_params  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libps3d_gpu.so")
+
+SVOR, VOR, VEL, SVEL, SVORTS, PRES, DELTA = range(7)
+(OP_FFTXYP2S, OP_FFTXYS2P, OP_DIFFX, OP_DIFFY, OP_FFTSINE, OP_FFTCOSINE,
+ OP_CENTRAL_DIFFZ, OP_COMBINE_SEMI, OP_DECOMPOSE_SEMI, OP_COMBINE_PHYS,
+ OP_DECOMPOSE_PHYS) = range(11)
+NDIAG = 19
+DIAG_NAMES = ["t", "dt", "ke", "en", "hel", "vortmax", "vortrms", "vorch",
+              "vormean_x", "vormean_y", "vormean_z", "ggmax", "umax", "vmax",
+              "wmax", "usggmax", "lsggmax", "rgmax", "prefactor"]
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class Ps3dError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("ps3d_gpu error %d: %s" % (code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libps3d_gpu.so; fails loudly when the HIP library is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("HIP library %s is not built: run `python -m ps3d_amd.build`" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.ps3d_gpu_last_error.restype = C.c_char_p
+        L.ps3d_gpu_version.restype = C.c_char_p
+        L.ps3d_gpu_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        L.ps3d_gpu_destroy.argtypes = [C.c_void_p]
+        L.ps3d_gpu_set_vorticity.argtypes = [C.c_void_p, _dp]
+        for n in ("ps3d_gpu_vor2vel", "ps3d_gpu_source", "ps3d_gpu_sync", "ps3d_gpu_derived"):
+            getattr(L, n).argtypes = [C.c_void_p]
+        L.ps3d_gpu_adapt.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_void_p]
+        L.ps3d_gpu_step.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double]
+        L.ps3d_gpu_advance_n.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_void_p]
+        L.ps3d_gpu_get.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ps3d_gpu_set.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ps3d_gpu_op.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.ps3d_gpu_fft1d.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        L.ps3d_gpu_rk_substep.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
+        L.ps3d_gpu_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        raise Ps3dError(st, lib().ps3d_gpu_last_error().decode())
+
+
+FFT1D_KIND = {"forfft": 0, "revfft": 1, "dct": 2, "dst": 3}
+
+
+def fft1d(kind, x):
+    """stafft transform (src/fft/stafft.f90) of Fortran x(m, n) given as a numpy
+    array of shape (n, m) (dct: (n+1, m)) in C order."""
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    ln, m = x.shape
+    n = ln - 1 if kind == "dct" else ln
+    _check(lib().ps3d_gpu_fft1d(FFT1D_KIND[kind], m, n, x))
+    return x
+
+
+class Ps3dGpu:
+    """One grid resident on the current HIP device (a ps3d_handle)."""
+
+    def __init__(self, params):
+        if not isinstance(params, Params):
+            params = _convert(params)
+        self.p = params
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self.shape = (self.nx, self.ny, self.nz + 1)
+        self.h = C.c_void_p()
+        _check(lib().ps3d_gpu_create(C.byref(params), C.byref(self.h)))
+        self.t = 0.0
+
+    def close(self):
+        if self.h:
+            lib().ps3d_gpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- setup (src/utils/utils.f90:136-184) ---
+    def set_vorticity(self, vor):
+        vor = np.ascontiguousarray(vor, dtype=np.float64)
+        if vor.shape != (3,) + self.shape:
+            raise ValueError("vorticity must have shape %s" % ((3,) + self.shape,))
+        _check(lib().ps3d_gpu_set_vorticity(self.h, vor))
+        self.t = 0.0
+
+    # --- operators ---
+    def vor2vel(self):
+        _check(lib().ps3d_gpu_vor2vel(self.h))
+
+    def source(self):
+        _check(lib().ps3d_gpu_source(self.h))
+
+    def adapt(self, t):
+        dt = C.c_double()
+        diag = np.zeros(NDIAG)
+        _check(lib().ps3d_gpu_adapt(self.h, t, C.byref(dt), diag.ctypes.data))
+        return dt.value, diag
+
+    def step(self, t, dt):
+        tt = C.c_double(t)
+        _check(lib().ps3d_gpu_step(self.h, C.byref(tt), dt))
+        return tt.value
+
+    def advance(self):
+        """One ps3d advance() (src/stepper/advance.f90:77-104); returns its diag row."""
+        return self.advance_n(1)[0]
+
+    def advance_n(self, nsteps):
+        tt = C.c_double(self.t)
+        diag = np.zeros((max(1, nsteps), NDIAG))
+        _check(lib().ps3d_gpu_advance_n(self.h, C.byref(tt), int(nsteps), diag.ctypes.data))
+        self.t = tt.value
+        return diag[:nsteps]
+
+    def sync(self):
+        _check(lib().ps3d_gpu_sync(self.h))
+
+    def last_timing(self):
+        ms = C.c_double()
+        n = C.c_int()
+        _check(lib().ps3d_gpu_last_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def get(self, which):
+        nv = 1 if which in (PRES, DELTA) else 3
+        out = np.zeros((3,) + self.shape) if nv == 3 else np.zeros(self.shape)
+        _check(lib().ps3d_gpu_get(self.h, which, out))
+        return out
+
+    def set(self, which, arr):
+        _check(lib().ps3d_gpu_set(self.h, which, np.ascontiguousarray(arr, dtype=np.float64)))
+
+    def op(self, op, fin):
+        fin = np.ascontiguousarray(fin, dtype=np.float64).reshape(self.shape)
+        out = np.zeros(self.shape)
+        _check(lib().ps3d_gpu_op(self.h, op, fin, out))
+        return out
+
+    def rk_substep(self, stage, dt, vdiss):
+        _check(lib().ps3d_gpu_rk_substep(self.h, int(stage), float(dt),
+                                         np.ascontiguousarray(vdiss, dtype=np.float64)))
+
+    # reference operator names (src/fft/sta3dfft.f90, src/inversion/inversion_utils.f90)
+    def fftxyp2s(self, fp):
+        return self.op(OP_FFTXYP2S, fp)
+
+    def fftxys2p(self, fs):
+        return self.op(OP_FFTXYS2P, fs)
+
+    def diffx(self, fs):
+        return self.op(OP_DIFFX, fs)
+
+    def diffy(self, fs):
+        return self.op(OP_DIFFY, fs)
+
+    def fftsine(self, fs):
+        return self.op(OP_FFTSINE, fs)
+
+    def fftcosine(self, fs):
+        return self.op(OP_FFTCOSINE, fs)
+
+    def central_diffz(self, fs):
+        return self.op(OP_CENTRAL_DIFFZ, fs)
+
+    def field_combine_semi_spectral(self, f):
+        return self.op(OP_COMBINE_SEMI, f)
+
+    def field_decompose_semi_spectral(self, f):
+        return self.op(OP_DECOMPOSE_SEMI, f)
+
+    def field_combine_physical(self, f):
+        return self.op(OP_COMBINE_PHYS, f)
+
+    def field_decompose_physical(self, f):
+        return self.op(OP_DECOMPOSE_PHYS, f)
+
+
+def _convert(p):
+    """Copy a ps3d_params-like ctypes structure (same fields) into Params."""
+    q = Params()
+    for name, _ in Params._fields_:
+        v = getattr(p, name)
+        if name in ("lower", "extent", "f_cor"):
+            getattr(q, name)[:] = list(v)
+        else:
+            setattr(q, name, v)
+    return q
