@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X pseudo-spectral step (FP64) on one GPU.
+
+Default workload: ps3d's plain FP64 time step on a 256^3 grid (2 pi x 2 pi x pi,
+periodic x, y, free-slip lids), 4-mode Beltrami initial vorticity with seeded
+phases plus 1e-3 seeded noise, impl-diff-rk4, 2/3-rule, nnu = 3, prediss = 30,
+pretype vorch, adaptive CFL/strain dt.  A "step" is one ps3d advance(): vor2vel,
+adapt, source and the four RK stages.  The grid is resident in HBM before the
+timed region starts.
+
+Prints one JSON line: steps/s (the metric), grid-point updates/s, the roofline
+of the dominant kernel (HIP events on the handle's stream), the whole-step
+algorithmic bandwidth, and the CPU baseline (the oracle, a C restatement of
+ps3d's unfused algorithm, timed on the host cores on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
+B_STEP_VOLUMES = 218       # algorithmic full-volume FP64 touches per RK4 step (fused floor)
+KIND_NAMES = ["zinv", "yinv", "xmain", "yfwd", "ztend", "yinv_adapt", "xadapt", "reduce", "mean"]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="grid size n^3")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def workload_params(mod, n):
+    import math
+    return mod.make_params(n, n, n, lower=(0.0, 0.0, 0.0), extent=(2 * math.pi, 2 * math.pi, math.pi),
+                           filtering="2/3-rule", stepper="impl-diff-rk4", nnu=3, prediss=30.0,
+                           pretype="vorch", length_scale="Kolmogorov", alpha=0.1, dt_fixed=0.0,
+                           limit=1.0e30)
+
+
+def read_traffic(n):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    counter summary (profiles/traffic_<kernel>_<n>.txt: a line 'bytes_per_launch <B>'),
+    or None."""
+    import glob
+    out = {}
+    for path in glob.glob(os.path.join(HERE, "profiles", "traffic_*_%d.txt" % n)):
+        kind = os.path.basename(path)[len("traffic_"):-len("_%d.txt" % n)]
+        with open(path) as f:
+            for line in f:
+                if line.startswith("bytes_per_launch"):
+                    out[kind] = float(line.split()[1])
+    return out
+
+
+def cpu_baseline(vor, n, steps, threads):
+    from oracle import oracle as O
+    ncores = threads if threads > 0 else (os.cpu_count() or 1)
+    O.set_threads(ncores)
+    o = O.Oracle(workload_params(O, n))
+    o.set_vorticity(vor)
+    t0 = time.perf_counter()
+    o.advance_n(steps)
+    el = time.perf_counter() - t0
+    o.close()
+    return {"value": steps / el, "unit": "steps/s", "cores": ncores, "kind": "port",
+            "sample": "%d step(s) of the same %d^3 workload through the C FP64 restatement of ps3d's "
+                      "unfused step (oracle/ps3d_oracle.c), OpenMP over %d host threads" % (steps, n, ncores)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the whole grid resident in one MI355X's HBM); "
+              "--gpus %d is not supported" % a.gpus, file=sys.stderr)
+        return 2
+    from ps3d_amd import gpu as G
+    from ps3d_amd import ics
+
+    n = a.n
+    vor = ics.beltrami_multimode(n, n, n, seed=1)
+    d = G.Ps3dGpu(workload_params(G, n))
+    d.set_vorticity(vor)
+    if a.warmup:
+        d.advance_n(a.warmup)
+    d.sync()
+    t0 = time.perf_counter()
+    diag = d.advance_n(a.steps)       # synchronises at the end
+    el = time.perf_counter() - t0
+    ev_ms, _ = d.last_timing()
+    steps_per_s = a.steps / el
+    npts = n * n * (n + 1)
+
+    # per-kernel durations (HIP events on the handle's stream), outside the timed region
+    import ctypes as C
+    import numpy as np
+    ms = np.zeros(9)
+    launches = np.zeros(9, dtype=np.int32)
+    nbytes = np.zeros(9)
+    st = G.lib().ps3d_gpu_profile_kernels(d.h, a.profile_steps, ms.ctypes.data_as(C.c_void_p),
+                                          launches.ctypes.data_as(C.c_void_p), nbytes.ctypes.data_as(C.c_void_p))
+    kernels = {}
+    if st == 0:
+        for k, name in enumerate(KIND_NAMES):
+            if launches[k]:
+                avg = ms[k] / launches[k] * 1e-3
+                kernels[name] = {"ms_per_step": ms[k] / a.profile_steps, "avg_launch_us": avg * 1e6,
+                                 "algo_bytes": nbytes[k],
+                                 "gbs": (nbytes[k] / avg / 1e9) if nbytes[k] > 0 else None}
+    dom = max((k for k in kernels if kernels[k]["algo_bytes"] > 0), key=lambda k: kernels[k]["ms_per_step"],
+              default=None)
+    traffic = read_traffic(n)
+    roofline = None
+    if dom:
+        achieved = kernels[dom]["gbs"] * 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                    "unit": "GB/s", "frac": achieved / PEAK_HBM,
+                    "traffic": traffic.get(dom)}
+    b_step = B_STEP_VOLUMES * 8.0 * npts
+    out = {
+        "metric": "RK steps/s at %d^3 FP64" % n,
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * el / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain FP64 Beltrami %d^3, impl-diff-rk4, 2/3-rule, adaptive CFL dt" % n,
+                   "grid": [n, n, n], "npts": npts, "ic": "4 Beltrami modes, seeded phases, 1e-3 noise (seed 1)"},
+        "grid_point_updates_per_s": npts * steps_per_s,
+        "event_ms_per_step": ev_ms / a.steps,
+        "step_roofline": {"bound": "hbm", "algo_bytes_per_step": b_step,
+                          "achieved": b_step * steps_per_s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                          "frac": b_step * steps_per_s / PEAK_HBM},
+        "roofline": roofline,
+        "kernels": kernels,
+        "t_end": float(diag[-1][0] + diag[-1][1]) if len(diag) else None,
+    }
+    d.close()
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(vor, n, a.cpu_steps, a.cpu_threads)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

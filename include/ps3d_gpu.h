@@ -166,6 +166,27 @@ int ps3d_gpu_fft1d(int kind, int m, int n, double *x);
  * which holds the tendency fixed across the four stages. */
 int ps3d_gpu_rk_substep(ps3d_handle *h, int stage, double dt, const double *vdiss);
 
+/* Kernel kinds of the fused step, for ps3d_gpu_profile_kernels. */
+enum ps3d_kernel_kind {
+    PS3D_K_ZINV = 0,      /* z pass of vor2vel                                   */
+    PS3D_K_YINV = 1,      /* y inverse, 6 fields                                 */
+    PS3D_K_XMAIN = 2,     /* x inverse, u x omega, x forward                     */
+    PS3D_K_YFWD = 3,      /* y forward, 3 fields                                 */
+    PS3D_K_ZTEND = 4,     /* z pass of the tendency + RK substep                 */
+    PS3D_K_YINV_ADAPT = 5,/* y inverse at the adapt stage (+ derivatives, sums)  */
+    PS3D_K_XADAPT = 6,    /* strain / CFL / vorticity diagnostics                */
+    PS3D_K_REDUCE = 7,    /* one-block reductions                                */
+    PS3D_K_MEAN = 8,      /* vorticity-mean restore + time update                */
+    PS3D_NKIND = 9
+};
+
+/* Runs nsteps of advance with a HIP event pair around every kernel on the
+ * handle's stream and reports, per kernel kind, the summed milliseconds, the
+ * number of launches and the algorithmic HBM bytes per launch (the bytes a pass
+ * must move: every field it reads and writes once).  Arrays hold PS3D_NKIND
+ * entries.  Not for the timed region: the events add gaps between kernels. */
+int ps3d_gpu_profile_kernels(ps3d_handle *h, int nsteps, double *ms, int *launches, double *bytes);
+
 /* Wait for all work queued on the handle's stream. */
 int ps3d_gpu_sync(ps3d_handle *h);
 

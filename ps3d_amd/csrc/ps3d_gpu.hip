@@ -68,7 +68,31 @@ struct ps3d_handle {
     double last_ms = 0.0;
     int last_steps = 0;
     std::vector<double> h_rkx, h_rky;
+    // per-kernel profiling (ps3d_gpu_profile_kernels)
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<int> prof_kind;
+    size_t prof_used = 0;
 };
+
+// Event pair around the next launch when profiling is on.
+static void prof_begin(ps3d_handle *h, int kind) {
+    if (!h->prof) return;
+    if (h->prof_used + 2 > h->prof_ev.size()) {
+        for (int i = 0; i < 64; ++i) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            h->prof_ev.push_back(e);
+        }
+    }
+    hipEventRecord(h->prof_ev[h->prof_used], h->stream);
+    h->prof_kind.push_back(kind);
+}
+static void prof_end(ps3d_handle *h) {
+    if (!h->prof) return;
+    hipEventRecord(h->prof_ev[h->prof_used + 1], h->stream);
+    h->prof_used += 2;
+}
 
 // ---------------------------------------------------------------------------
 // host tables (the reference's formulas, evaluated in the same order)
@@ -168,6 +192,7 @@ static void free_handle(ps3d_handle *h) {
     hipFree(h->twx);
     hipFree(h->twy);
     hipFree(h->twz);
+    for (hipEvent_t e : h->prof_ev) hipEventDestroy(e);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -422,7 +447,9 @@ static int run_mean(ps3d_handle *h, int mode, int advance_time) {
     a.scal = h->scal;
     a.mode = mode;
     a.advance_time = advance_time;
+    prof_begin(h, PS3D_K_MEAN);
     HIPCHK(launch_mean(a, h->g.nz, h->stream));
+    prof_end(h);
     return 0;
 }
 
@@ -436,7 +463,9 @@ static int inverse_adapt_stage(ps3d_handle *h, bool project) {
         z.svor_out[1] = h->svor[1];
     }
     for (int c = 0; c < 3; ++c) z.mid[c] = h->mid[c];
+    prof_begin(h, PS3D_K_ZINV);
     HIPCHK(launch_zinv(h->zl, z, h->stream));
+    prof_end(h);
     LineArgs y = line_args(h, 1, LM_INV_Y1);
     for (int c = 0; c < 3; ++c) {
         y.in[c] = h->svel[c];
@@ -448,9 +477,13 @@ static int inverse_adapt_stage(ps3d_handle *h, bool project) {
         y.out[9 + c] = h->grad[c];
     }
     y.part = h->party;
+    prof_begin(h, PS3D_K_YINV_ADAPT);
     int st = run_line(h, y);
+    prof_end(h);
     if (st) return st;
+    prof_begin(h, PS3D_K_REDUCE);
     HIPCHK(launch_fin_y(fin_args(h), h->stream));
+    prof_end(h);
     return 0;
 }
 
@@ -464,9 +497,13 @@ static int x_adapt(ps3d_handle *h, int phys_in) {
         x.in[9 + c] = h->mid[c];
     }
     x.part = h->partx;
+    prof_begin(h, PS3D_K_XADAPT);
     int st = run_line(h, x);
+    prof_end(h);
     if (st) return st;
+    prof_begin(h, PS3D_K_REDUCE);
     HIPCHK(launch_fin_x(fin_args(h), h->stream));
+    prof_end(h);
     return 0;
 }
 
@@ -479,9 +516,14 @@ static int products(ps3d_handle *h, int phys_in, double *const *dst) {
         x.in[3 + c] = h->vor[c];
         x.out[c] = dst[c];
     }
+    prof_begin(h, PS3D_K_XMAIN);
     int st = run_line(h, x);
+    prof_end(h);
     if (st) return st;
-    return line_fields(h, 1, LM_FWD, 3, dst, dst);
+    prof_begin(h, PS3D_K_YFWD);
+    st = line_fields(h, 1, LM_FWD, 3, dst, dst);
+    prof_end(h);
+    return st;
 }
 
 // One fused RK stage (impl_rk4) or cn2 iteration.  stage 1 includes adapt.
@@ -492,14 +534,21 @@ static int fused_stage(ps3d_handle *h, int mode, int stage) {
         if ((st = x_adapt(h, 0))) return st;
     } else {
         ZInvArgs z = zinv_args(h);
+        prof_begin(h, PS3D_K_ZINV);
         HIPCHK(launch_zinv(h->zl, z, h->stream));
+        prof_end(h);
         double *in[6] = {h->svel[0], h->svel[1], h->svel[2], h->vor[0], h->vor[1], h->vor[2]};
         double *out[6] = {h->vel[0], h->vel[1], h->vel[2], h->vor[0], h->vor[1], h->vor[2]};
-        if ((st = line_fields(h, 1, LM_INV, 6, in, out))) return st;
+        prof_begin(h, PS3D_K_YINV);
+        st = line_fields(h, 1, LM_INV, 6, in, out);
+        prof_end(h);
+        if (st) return st;
     }
     if ((st = products(h, 0, h->vel))) return st;
     ZTendArgs t = ztend_args(h, mode, stage, h->vel);
+    prof_begin(h, PS3D_K_ZTEND);
     HIPCHK(launch_ztend(h->zl, t, h->stream));
+    prof_end(h);
     return 0;
 }
 
@@ -750,6 +799,50 @@ int ps3d_gpu_rk_substep(ps3d_handle *h, int stage, double dt, const double *vdis
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     hipFree(dv);
     if (e != hipSuccess) return fail(PS3D_E_HIP, std::string("rk_substep: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int ps3d_gpu_profile_kernels(ps3d_handle *h, int nsteps, double *ms, int *launches, double *bytes) {
+    if (!h || nsteps < 1) return fail(PS3D_E_ARG, "invalid argument");
+    if (!h->have_state) return fail(PS3D_E_STATE, "profile before set_vorticity");
+    int st;
+    if ((st = ensure_diag(h, nsteps))) return st;
+    double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(h->scal + SC_STEP, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    h->prof = true;
+    h->prof_used = 0;
+    h->prof_kind.clear();
+    for (int s = 0; s < nsteps && !st; ++s) st = one_step(h);
+    h->prof = false;
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < PS3D_NKIND; ++k) {
+        if (ms) ms[k] = 0.0;
+        if (launches) launches[k] = 0;
+    }
+    for (size_t i = 0; i < h->prof_kind.size(); ++i) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, h->prof_ev[2 * i], h->prof_ev[2 * i + 1]));
+        const int k = h->prof_kind[i];
+        if (ms) ms[k] += t;
+        if (launches) launches[k] += 1;
+    }
+    if (bytes) {
+        // algorithmic bytes per launch: each field a pass reads and writes, once
+        const double vol = 8.0 * (double)h->g.nx * h->g.ny * (h->g.nz + 1);
+        const bool cn2 = h->p.stepper == PS3D_STEPPER_CN2;
+        // z inverse: 3 R + 6 W; at the adapt stage + 2 W (projected svor) + 3 W (midpoints)
+        bytes[PS3D_K_ZINV] = (cn2 ? (14.0 + 2 * 9.0) / 3.0 : (14.0 + 3 * 9.0) / 4.0) * vol;
+        bytes[PS3D_K_YINV] = 12.0 * vol;        // 6 R + 6 W
+        bytes[PS3D_K_XMAIN] = 9.0 * vol;        // 6 R + 3 W
+        bytes[PS3D_K_YFWD] = 6.0 * vol;         // 3 R + 3 W
+        // z forward + update, per stage: rk4 15, 15, 15, 9 ; cn2 12, 9, 9
+        bytes[PS3D_K_ZTEND] = (cn2 ? 10.0 : 13.5) * vol;
+        bytes[PS3D_K_YINV_ADAPT] = 21.0 * vol;  // 9 R + 12 W
+        bytes[PS3D_K_XADAPT] = 12.0 * vol;      // 12 R
+        bytes[PS3D_K_REDUCE] = 0.0;
+        bytes[PS3D_K_MEAN] = 0.0;
+    }
     return 0;
 }
 

@@ -62,6 +62,7 @@ def lib():
         L.ps3d_gpu_fft1d.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
         L.ps3d_gpu_rk_substep.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
         L.ps3d_gpu_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.ps3d_gpu_profile_kernels.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 

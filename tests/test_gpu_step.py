@@ -1,0 +1,174 @@
+"""Whole-step parity of the HIP path with the CPU oracle: the reference's
+advance (vor2vel, adapt, source, impl_rk4 / cn2 step) from the same seeded
+initial vorticity, field for field and for the diagnostics series."""
+import numpy as np
+import pytest
+
+from ps3d_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+PI = np.pi
+# FP64 trajectory bar: relative max-norm over each field and each diagnostic
+TOL = 1.0e-12
+
+
+@pytest.fixture(scope="module")
+def G():
+    from ps3d_amd import gpu
+    gpu.lib()
+    return gpu
+
+
+def box(n):
+    return dict(nx=n, ny=n, nz=n, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI))
+
+
+def config1(**kw):
+    """The 32^3 Beltrami workload: impl-diff-rk4, 2/3-rule, nnu=3, prediss=30,
+    pretype vorch, Kolmogorov, fixed dt = 0.01."""
+    d = dict(filtering="2/3-rule", stepper="impl-diff-rk4", nnu=3, prediss=30.0, pretype="vorch",
+             length_scale="Kolmogorov", alpha=0.1, dt_fixed=0.01, limit=1.0e30)
+    d.update(kw)
+    return d
+
+
+def make_pair(oracle_mod, G, n, **kw):
+    g = box(n)
+    po = oracle_mod.make_params(g["nx"], g["ny"], g["nz"], lower=g["lower"], extent=g["extent"], **kw)
+    pg = G.make_params(g["nx"], g["ny"], g["nz"], lower=g["lower"], extent=g["extent"], **kw)
+    return oracle_mod.Oracle(po), G.Ps3dGpu(pg)
+
+
+def field_rel(a, b):
+    return max(np.abs(a[c] - b[c]).max() / np.abs(b[c]).max() for c in range(a.shape[0]))
+
+
+def diag_rel(da, db, cols):
+    out = {}
+    for j in cols:
+        scale = max(np.abs(db[:, j]).max(), 1e-300)
+        out[j] = np.abs(da[:, j] - db[:, j]).max() / scale
+    return out
+
+
+# diagnostics compared: all but the vorticity means (O(1e-5) noise means whose
+# relative error is dominated by roundoff of O(1) sums)
+DIAG_COLS = [0, 1, 2, 3, 4, 5, 6, 7, 11, 12, 13, 14, 15, 16, 17, 18]
+
+
+def test_setup_matches_oracle(oracle_mod, G):
+    o, d = make_pair(oracle_mod, G, 32, **config1())
+    vor = ics.beltrami_multimode(32, 32, 32, seed=1)
+    o.set_vorticity(vor)
+    d.set_vorticity(vor)
+    for which in (G.SVOR, G.VEL, G.VOR, G.SVEL):
+        r = field_rel(d.get(which), o.get(which))
+        print("setup field", which, r, flush=True)
+        assert r < 1e-13, which
+    d.close()
+
+
+def test_api_path_three_steps(oracle_mod, G):
+    """vor2vel / adapt / source / step through the C ABI, one call at a time,
+    against the same calls on the oracle."""
+    o, d = make_pair(oracle_mod, G, 32, **config1())
+    vor = ics.beltrami_multimode(32, 32, 32, seed=1)
+    o.set_vorticity(vor)
+    d.set_vorticity(vor)
+    t_o = t_d = 0.0
+    for k in range(3):
+        print("api step", k, flush=True)
+        o.vor2vel()
+        d.vor2vel()
+        dto, rowo = o.adapt(t_o)
+        dtd, rowd = d.adapt(t_d)
+        r = diag_rel(rowd[None], rowo[None], DIAG_COLS)
+        assert max(r.values()) < 1e-12, r
+        o.source()
+        d.source()
+        assert field_rel(d.get(G.SVORTS), o.get(oracle_mod.SVORTS)) < 1e-12
+        t_o = o.step(t_o, dto)
+        t_d = d.step(t_d, dtd)
+        assert abs(t_o - t_d) < 1e-15
+        assert field_rel(d.get(G.SVOR), o.get(oracle_mod.SVOR)) < 1e-12
+    d.close()
+
+
+@pytest.mark.parametrize("nsteps", [100])
+def test_beltrami32_fixed_dt_trajectory(oracle_mod, G, nsteps):
+    """32^3 Beltrami, fixed dt = 0.01, 100 steps: every field and the
+    diagnostics series within 1e-12 relative max-norm of the oracle."""
+    o, d = make_pair(oracle_mod, G, 32, **config1())
+    vor = ics.beltrami_multimode(32, 32, 32, seed=1)
+    o.set_vorticity(vor)
+    d.set_vorticity(vor)
+    print("oracle run", flush=True)
+    do = o.advance_n(nsteps)
+    print("gpu run", flush=True)
+    dd = d.advance_n(nsteps)
+    r = diag_rel(dd, do, DIAG_COLS)
+    print("diag rel", r, flush=True)
+    assert max(r.values()) < TOL, r
+    o.vor2vel()
+    d.vor2vel()
+    for which in (G.SVOR, G.VEL, G.VOR):
+        fr = field_rel(d.get(which), o.get(which))
+        print("field", which, fr, flush=True)
+        assert fr < TOL, which
+    d.close()
+
+
+def test_adaptive_hou_li_trajectory(oracle_mod, G):
+    """ps3d's defaults: Hou & Li filter, adaptive dt, roll-mean-max-strain."""
+    kw = dict(filtering="Hou & Li", pretype="roll-mean-max-strain", prediss=10.0, nnu=3, alpha=0.1,
+              dt_fixed=0.0, limit=1.0e30)
+    o, d = make_pair(oracle_mod, G, 32, **kw)
+    vor = ics.beltrami_multimode(32, 32, 32, seed=2)
+    o.set_vorticity(vor)
+    d.set_vorticity(vor)
+    do = o.advance_n(30)
+    dd = d.advance_n(30)
+    r = diag_rel(dd, do, DIAG_COLS)
+    print("diag rel", r, flush=True)
+    assert max(r.values()) < TOL, r
+    assert field_rel(d.get(G.SVOR), o.get(oracle_mod.SVOR)) < TOL
+    d.close()
+
+
+def test_cn2_trajectory(oracle_mod, G):
+    """The cn2 stepper of the shipped examples (examples/beltrami_32.config)."""
+    kw = dict(filtering="Hou & Li", stepper="cn2", pretype="vorch", prediss=30.0, nnu=3, alpha=0.1,
+              dt_fixed=0.0, limit=1.0e30)
+    o, d = make_pair(oracle_mod, G, 32, **kw)
+    vor = ics.beltrami_multimode(32, 32, 32, seed=3)
+    o.set_vorticity(vor)
+    d.set_vorticity(vor)
+    do = o.advance_n(10)
+    dd = d.advance_n(10)
+    r = diag_rel(dd, do, DIAG_COLS)
+    print("diag rel", r, flush=True)
+    assert max(r.values()) < TOL, r
+    assert field_rel(d.get(G.SVOR), o.get(oracle_mod.SVOR)) < TOL
+    d.close()
+
+
+def test_fused_advance_equals_api_path(G):
+    """advance_n (fused, no host round trips) and the one-call-at-a-time path
+    give the same state."""
+    p = G.make_params(**box(32), **config1())
+    a, b = G.Ps3dGpu(p), G.Ps3dGpu(p)
+    vor = ics.beltrami_multimode(32, 32, 32, seed=4)
+    a.set_vorticity(vor)
+    b.set_vorticity(vor)
+    a.advance_n(2)
+    t = 0.0
+    for k in range(2):
+        b.vor2vel()
+        dt, _ = b.adapt(t)
+        b.source()
+        t = b.step(t, dt)
+    assert abs(a.t - t) < 1e-15
+    assert field_rel(a.get(G.SVOR), b.get(G.SVOR)) < 1e-13
+    a.close()
+    b.close()
