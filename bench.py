@@ -37,7 +37,22 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy (y = x + 0) of MIB mebibytes of FP64, 5 times, to "
+                         "calibrate the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * 1024 * 1024 // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    for _ in range(5):
+        y = x + 0.0
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 8, "copy_calibration_bytes_written": n * 8,
+                      "launches": 5, "check": float(y[0].item())}))
+    return 0
 
 
 def workload_params(mod, n):
@@ -80,6 +95,8 @@ def cpu_baseline(vor, n, steps, threads):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        return copy_calibration(a.copy_calibration)
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (the whole grid resident in one MI355X's HBM); "
               "--gpus %d is not supported" % a.gpus, file=sys.stderr)
