@@ -40,6 +40,8 @@ struct ZTabs {
 struct ZLaunch {
     int zpt, blocks;
     size_t lds;
+    int nz;
+    int threads;   // TQ * QPB
 };
 
 struct ZInvArgs {

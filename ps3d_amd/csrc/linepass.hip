@@ -7,16 +7,35 @@
 // of two levels; the Hermitian real form of stafft (src/fft/stafft.f90:55-58,
 // 196-403) keeps the same packing: slot j of line p holds (level 2p, level 2p+1).
 #include "kernels.hpp"
+#include "lfft.hpp"
 
 namespace ps3d {
 
-constexpr int kEpt = 4;        // complex elements per thread held in registers
-constexpr int kLineThreads = 512;
+constexpr int kEpt = 2;        // complex elements per thread held in registers
+constexpr int kLineThreads = 1024;
 
 struct Tile {
     int n, ZC, nl, LS, zce, z0, other;
     long base;
+    const double2 *twl;   // LDS copy of the twiddles (compile-time lengths)
 };
+
+// z depth of a tile for a line length: n*ZC <= 4096 doubles, 2 <= ZC <= 16, even
+__host__ __device__ constexpr int zc_for(int n) {
+    return ((4096 / n > 16 ? 16 : (4096 / n < 2 ? 2 : 4096 / n)) & ~1);
+}
+
+// Register-holding modes run 1024 threads per tile and use the radix-8 plans to
+// stay within 128 VGPRs; transform-only modes use the radix-16 plans.
+template <int N, bool REG = false>
+__device__ inline double2 *lfft(const Tile &t, const LineArgs &a, double2 *A, double2 *B, int dir) {
+    if constexpr (N == 0)
+        return fft_lds(A, B, t.nl, t.LS, a.P, a.tw, dir);
+    else if constexpr (REG && CPlan8<N>::ok)
+        return cfft_lds<N, CPlan8<N>>(A, B, t.nl, t.LS, t.twl, dir);
+    else
+        return cfft_lds<N>(A, B, t.nl, t.LS, t.twl, dir);
+}
 
 __device__ inline Tile make_tile(const LineArgs &a) {
     Tile t;
@@ -30,6 +49,7 @@ __device__ inline Tile make_tile(const LineArgs &a) {
     const int rem = a.g.nz + 1 - t.z0;
     t.zce = rem < a.ZC ? rem : a.ZC;
     t.base = (long)t.other * a.ostride + t.z0;
+    t.twl = nullptr;
     return t;
 }
 
@@ -55,6 +75,7 @@ __device__ inline void tile_store(const Tile &t, const LineArgs &a, double *__re
 
 // Hermitian pair -> complex spectrum, then inverse FFT.  Returns result buffer,
 // values to be scaled by 1/n.
+template <int N, bool REG = false>
 __device__ inline double2 *tile_inverse(const Tile &t, const LineArgs &a, double2 *A, double2 *B) {
     const int n = t.n, h = n / 2;
     const double sn = sqrt((double)n);
@@ -72,12 +93,13 @@ __device__ inline double2 *tile_inverse(const Tile &t, const LineArgs &a, double
         }
     }
     __syncthreads();
-    return fft_lds(A, B, t.nl, t.LS, a.P, a.tw, +1);
+    return lfft<N, REG>(t, a, A, B, +1);
 }
 
 // Forward FFT then split into the Hermitian pair, in the returned buffer, scaled.
+template <int N, bool REG = false>
 __device__ inline double2 *tile_forward(const Tile &t, const LineArgs &a, double2 *A, double2 *B) {
-    double2 *R = fft_lds(A, B, t.nl, t.LS, a.P, a.tw, -1);
+    double2 *R = lfft<N, REG>(t, a, A, B, -1);
     const int n = t.n, h = n / 2;
     const double sc = 1.0 / sqrt((double)n);
     const int tot = t.nl * (h + 1);
@@ -152,6 +174,7 @@ __device__ inline int reg_j(const Tile &t, int k) { return (threadIdx.x + k * bl
 
 // Load one field, transform it to physical space along the line (unless the input
 // is already physical), and keep it in registers.
+template <int N>
 __device__ inline void field_phys(const Tile &t, const LineArgs &a, const double *src, double2 *A, double2 *B,
                                   double2 (&r)[kEpt], bool diff) {
     tile_load(t, a, src, A);
@@ -159,7 +182,7 @@ __device__ inline void field_phys(const Tile &t, const LineArgs &a, const double
     double2 *S = A, *O = B;
     if (diff) {
         if (a.phys_in) {
-            S = tile_forward(t, a, A, B);
+            S = tile_forward<N, true>(t, a, A, B);
             O = (S == A) ? B : A;
         }
         tile_diff(t, a, S, O);
@@ -171,17 +194,18 @@ __device__ inline void field_phys(const Tile &t, const LineArgs &a, const double
         __syncthreads();
         return;
     }
-    double2 *R = tile_inverse(t, a, S, O);
+    double2 *R = tile_inverse<N, true>(t, a, S, O);
     to_regs(t, R, r, 1.0 / (double)t.n);
     __syncthreads();
 }
 
 // registers -> forward transform -> global
+template <int N>
 __device__ inline void fwd_store(const Tile &t, const LineArgs &a, double2 *A, double2 *B, const double2 (&r)[kEpt],
                                  double *dst) {
     from_regs(t, A, r);
     __syncthreads();
-    double2 *R = tile_forward(t, a, A, B);
+    double2 *R = tile_forward<N, true>(t, a, A, B);
     tile_store(t, a, dst, R, 1.0);
     __syncthreads();
 }
@@ -270,23 +294,28 @@ __device__ inline double jacobi_absmax(double s11, double s12, double s13, doubl
     return fmax(fmax(fabs(D[0]), fabs(D[1])), fabs(D[2]));
 }
 
-__global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
+// Transform-only modes (forward, inverse, derivative) of nf fields; the global
+// loads of field f+1 are issued before the FFT of field f.
+template <int N>
+__global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
     extern __shared__ double2 lsmem[];
-    const Tile t = make_tile(a);
+    Tile t = make_tile(a);
     const size_t tb = (size_t)t.nl * t.LS;
     double2 *A = lsmem, *B = lsmem + tb;
-    double *scratch = reinterpret_cast<double *>(lsmem);
-    const int nz = a.g.nz;
-
-    if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF) {
+    if constexpr (N > 0) {
+        double2 *twl = B + tb;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = a.tw[i];
+        t.twl = twl;
+    }
+    if (a.mode == LM_DIFF || N == 0) {
         for (int f = 0; f < a.nf; ++f) {
             tile_load(t, a, a.in[f], A);
             __syncthreads();
             if (a.mode == LM_FWD) {
-                double2 *R = tile_forward(t, a, A, B);
+                double2 *R = tile_forward<N>(t, a, A, B);
                 tile_store(t, a, a.out[f], R, 1.0);
             } else if (a.mode == LM_INV) {
-                double2 *R = tile_inverse(t, a, A, B);
+                double2 *R = tile_inverse<N>(t, a, A, B);
                 tile_store(t, a, a.out[f], R, 1.0 / (double)t.n);
             } else {
                 tile_diff(t, a, A, B);
@@ -296,6 +325,52 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
         }
         return;
     }
+    if constexpr (N > 0) {
+        constexpr int ZC = zc_for(N), PER = N * ZC / 256;
+        double pre[PER];
+        auto gload = [&](const double *__restrict__ src) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int e = threadIdx.x + k * 256, zz = e % ZC, j = e / ZC;
+                pre[k] = zz < t.zce ? src[t.base + (long)j * a.estride + zz] : 0.0;
+            }
+        };
+        gload(a.in[0]);
+        for (int f = 0; f < a.nf; ++f) {
+            double *d = reinterpret_cast<double *>(A);
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int e = threadIdx.x + k * 256, zz = e % ZC, j = e / ZC;
+                d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = pre[k];
+            }
+            __syncthreads();
+            if (f + 1 < a.nf) gload(a.in[f + 1]);
+            if (a.mode == LM_FWD) {
+                double2 *R = tile_forward<N>(t, a, A, B);
+                tile_store(t, a, a.out[f], R, 1.0);
+            } else {
+                double2 *R = tile_inverse<N>(t, a, A, B);
+                tile_store(t, a, a.out[f], R, 1.0 / (double)t.n);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Modes that keep physical values in registers (products, adapt diagnostics).
+template <int N>
+__global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
+    extern __shared__ double2 lsmem[];
+    Tile t = make_tile(a);
+    const size_t tb = (size_t)t.nl * t.LS;
+    double2 *A = lsmem, *B = lsmem + tb;
+    if constexpr (N > 0) {
+        double2 *twl = B + tb;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = a.tw[i];
+        t.twl = twl;
+    }
+    double *scratch = reinterpret_cast<double *>(lsmem);
+    const int nz = a.g.nz;
 
     if (a.mode == LM_INV_Y1) {
         // u, v, w with their y derivatives; xi, eta, zeta; optional midpoints.
@@ -310,12 +385,12 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
             tile_load(t, a, a.in[c], A);
             __syncthreads();
             tile_diff(t, a, A, B);            // y derivative in B
-            double2 *R = tile_inverse(t, a, B, A);
+            double2 *R = tile_inverse<N, true>(t, a, B, A);
             tile_store(t, a, a.out[9 + c], R, 1.0 / (double)t.n);
             __syncthreads();
             tile_load(t, a, a.in[c], A);
             __syncthreads();
-            R = tile_inverse(t, a, A, B);
+            R = tile_inverse<N, true>(t, a, A, B);
             tile_store(t, a, a.out[c], R, 1.0 / (double)t.n);
             to_regs(t, R, U[c], 1.0 / (double)t.n);
             __syncthreads();
@@ -324,7 +399,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
         for (int c = 0; c < 3; ++c) {
             tile_load(t, a, a.in[3 + c], A);
             __syncthreads();
-            double2 *R = tile_inverse(t, a, A, B);
+            double2 *R = tile_inverse<N, true>(t, a, A, B);
             tile_store(t, a, a.out[3 + c], R, 1.0 / (double)t.n);
             double2 W[kEpt];
             to_regs(t, R, W, 1.0 / (double)t.n);
@@ -351,7 +426,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
             for (int c = 0; c < 3; ++c) {
                 tile_load(t, a, a.in[6 + c], A);
                 __syncthreads();
-                double2 *R = tile_inverse(t, a, A, B);
+                double2 *R = tile_inverse<N, true>(t, a, A, B);
                 tile_store(t, a, a.out[6 + c], R, 1.0 / (double)t.n);
                 __syncthreads();
             }
@@ -377,25 +452,25 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
     if (a.mode == LM_XMAIN) {
         // u x omega in flux form (inversion.f90:316-352) with omega + f_cor
         double2 u[kEpt], v[kEpt], w[kEpt], r[kEpt], q[kEpt], p[kEpt];
-        field_phys(t, a, a.in[0], A, B, u, false);
-        field_phys(t, a, a.in[1], A, B, v, false);
-        field_phys(t, a, a.in[2], A, B, w, false);
+        field_phys<N>(t, a, a.in[0], A, B, u, false);
+        field_phys<N>(t, a, a.in[1], A, B, v, false);
+        field_phys<N>(t, a, a.in[2], A, B, w, false);
         double2 o[kEpt];
-        field_phys(t, a, a.in[3], A, B, o, false);   // xi
+        field_phys<N>(t, a, a.in[3], A, B, o, false);   // xi
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
             const double xr = o[k].x + a.fcor[0], xi = o[k].y + a.fcor[0];
             r[k] = make_double2(-v[k].x * xr, -v[k].y * xi);
             q[k] = make_double2(w[k].x * xr, w[k].y * xi);
         }
-        field_phys(t, a, a.in[4], A, B, o, false);   // eta
+        field_phys<N>(t, a, a.in[4], A, B, o, false);   // eta
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
             const double er = o[k].x + a.fcor[1], ei = o[k].y + a.fcor[1];
             r[k] = make_double2(u[k].x * er + r[k].x, u[k].y * ei + r[k].y);
             p[k] = make_double2(-w[k].x * er, -w[k].y * ei);
         }
-        field_phys(t, a, a.in[5], A, B, o, false);   // zeta
+        field_phys<N>(t, a, a.in[5], A, B, o, false);   // zeta
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
             const double zr = o[k].x + a.fcor[2], zi = o[k].y + a.fcor[2];
@@ -403,9 +478,9 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
             p[k] = make_double2(v[k].x * zr + p[k].x, v[k].y * zi + p[k].y);
         }
         // r = u eta - v xi ; q = w xi - u zeta ; p = v zeta - w eta  -> forward in x
-        fwd_store(t, a, A, B, r, a.out[0]);
-        fwd_store(t, a, A, B, q, a.out[1]);
-        fwd_store(t, a, A, B, p, a.out[2]);
+        fwd_store<N>(t, a, A, B, r, a.out[0]);
+        fwd_store<N>(t, a, A, B, q, a.out[1]);
+        fwd_store<N>(t, a, A, B, p, a.out[2]);
         return;
     }
 
@@ -417,29 +492,29 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
         double umax = -INFINITY, vmax = -INFINITY, wmax = -INFINITY, w2max = 0.0;
         double gg = 2.220446049250313e-16, usgg = 0.0, lsgg = 0.0, l1 = 0.0, l2 = 0.0;
         double2 ux[kEpt], uy[kEpt], vy[kEpt], wx[kEpt], wy[kEpt], xi[kEpt], et[kEpt], ze[kEpt];
-        field_phys(t, a, a.in[0], A, B, ux, false);   // u (for umax), then du/dx
+        field_phys<N>(t, a, a.in[0], A, B, ux, false);   // u (for umax), then du/dx
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h)
                 if (reg_valid(t, k, h, nz)) umax = fmax(umax, h ? ux[k].y : ux[k].x);
-        field_phys(t, a, a.in[1], A, B, vy, false);
+        field_phys<N>(t, a, a.in[1], A, B, vy, false);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h)
                 if (reg_valid(t, k, h, nz)) vmax = fmax(vmax, h ? vy[k].y : vy[k].x);
-        field_phys(t, a, a.in[2], A, B, wx, false);
+        field_phys<N>(t, a, a.in[2], A, B, wx, false);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h)
                 if (reg_valid(t, k, h, nz)) wmax = fmax(wmax, h ? wx[k].y : wx[k].x);
-        field_phys(t, a, a.in[0], A, B, ux, true);    // du/dx
-        field_phys(t, a, a.in[2], A, B, wx, true);    // dw/dx
-        field_phys(t, a, a.in[6], A, B, uy, false);   // du/dy
-        field_phys(t, a, a.in[7], A, B, vy, false);   // dv/dy
-        field_phys(t, a, a.in[8], A, B, wy, false);   // dw/dy
-        field_phys(t, a, a.in[3], A, B, xi, false);
-        field_phys(t, a, a.in[4], A, B, et, false);
-        field_phys(t, a, a.in[5], A, B, ze, false);
+        field_phys<N>(t, a, a.in[0], A, B, ux, true);    // du/dx
+        field_phys<N>(t, a, a.in[2], A, B, wx, true);    // dw/dx
+        field_phys<N>(t, a, a.in[6], A, B, uy, false);   // du/dy
+        field_phys<N>(t, a, a.in[7], A, B, vy, false);   // dv/dy
+        field_phys<N>(t, a, a.in[8], A, B, wy, false);   // dw/dy
+        field_phys<N>(t, a, a.in[3], A, B, xi, false);
+        field_phys<N>(t, a, a.in[4], A, B, et, false);
+        field_phys<N>(t, a, a.in[5], A, B, ze, false);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h) {
@@ -458,9 +533,9 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
             }
         // characteristic vorticity <|w|^2>/<|w|> over cells with |w| > w_rms
         // (field_diagnostics.f90:501-545), from the z-midpoint vorticity
-        field_phys(t, a, a.in[9], A, B, xi, false);
-        field_phys(t, a, a.in[10], A, B, et, false);
-        field_phys(t, a, a.in[11], A, B, ze, false);
+        field_phys<N>(t, a, a.in[9], A, B, xi, false);
+        field_phys<N>(t, a, a.in[10], A, B, et, false);
+        field_phys<N>(t, a, a.in[11], A, B, ze, false);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h) {
@@ -500,19 +575,45 @@ __global__ void __launch_bounds__(kLineThreads) k_line(LineArgs a) {
 
 size_t line_lds_bytes(int n, int ZC) {
     const size_t tb = (size_t)(ZC / 2) * (n + 1) * sizeof(double2);
-    size_t need = 2 * tb;
+    size_t need = 2 * tb + (size_t)n * sizeof(double2);
     const size_t red = kLineThreads * sizeof(double);
     return need > red ? need : red;
 }
 
+#define PS3D_LINE_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+
 hipError_t configure_linepass() {
-    return hipFuncSetAttribute((const void *)k_line, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    const int lds = 160 * 1024;
+    hipError_t e = hipSuccess;
+#define CFG(N_)                                                                                              \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_simple<N_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    CFG(0)
+    PS3D_LINE_SIZES(CFG)
+#undef CFG
+    return e;
+}
+
+template <int N>
+static void launch_line_t(const LineArgs &a, hipStream_t stream) {
+    const int blocks = a.nother * a.nzt;
+    const size_t lds = line_lds_bytes(a.n, a.ZC);
+    if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF)
+        hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks), dim3(256), lds, stream, a);
+    else
+        hipLaunchKernelGGL(k_line_reg<N>, dim3(blocks), dim3(kLineThreads), lds, stream, a);
 }
 
 hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
     (void)nthreads;
-    const int blocks = a.nother * a.nzt;
-    hipLaunchKernelGGL(k_line, dim3(blocks), dim3(kLineThreads), line_lds_bytes(a.n, a.ZC), stream, a);
+    // compile-time plans need the standard tile depth
+    const bool std_zc = a.ZC == zc_for(a.n);
+    switch (std_zc ? a.n : 0) {
+#define CASE(N_) case N_: launch_line_t<N_>(a, stream); break;
+        PS3D_LINE_SIZES(CASE)
+#undef CASE
+        default: launch_line_t<0>(a, stream); break;
+    }
     return hipGetLastError();
 }
 

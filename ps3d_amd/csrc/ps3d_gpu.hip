@@ -268,18 +268,25 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
         (st = upload(&h->twz, twiddles(2 * nz))))
         return st;
 
-    // z launch shape: TQ threads per quad, ZPT levels per thread, 256 threads per block
+    // z launch shape: TQ threads per quad (a multiple of 64) own ZPT levels each;
+    // small columns share a block (QPB quads), long ones take up to 512 threads
     const int nz1 = nz + 1;
+    auto up64 = [](int v) { return (v + 63) / 64 * 64; };
     if (nz1 <= 64) { h->TQ = 64; h->zl.zpt = 1; }
     else if (nz1 <= 128) { h->TQ = 128; h->zl.zpt = 1; }
-    else if (nz1 <= 256) { h->TQ = 256; h->zl.zpt = 1; }
-    else if (nz1 <= 512) { h->TQ = 256; h->zl.zpt = 2; }
-    else if (nz1 <= 768) { h->TQ = 256; h->zl.zpt = 3; }
-    else { h->TQ = 256; h->zl.zpt = 5; }
-    h->QPB = 256 / h->TQ;
+    else {
+        int zpt = 1;
+        while (up64((nz1 + zpt - 1) / zpt) > 512) ++zpt;
+        h->zl.zpt = zpt;
+        h->TQ = up64((nz1 + zpt - 1) / zpt);
+    }
+    if (h->zl.zpt > 3) return fail(PS3D_E_UNSUPPORTED, "nz too large");
+    h->QPB = h->TQ >= 256 ? 1 : 256 / h->TQ;
+    h->zl.threads = h->TQ * h->QPB;
     h->nquads = (nx / 2 + 1) * (ny / 2 + 1);
     h->zl.blocks = (h->nquads + h->QPB - 1) / h->QPB;
     h->zl.lds = zpass_lds_bytes(h->QPB, nz);
+    h->zl.nz = nz;
     if (h->zl.lds > 160 * 1024) return fail(PS3D_E_UNSUPPORTED, "nz too large for the LDS z transform");
 
     h->ly = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs);

@@ -12,6 +12,7 @@
 // src/fft/stafft.f90:410-550) go through LDS as complex FFTs of length 2nz of the
 // odd/even extension, two columns per complex line.
 #include "kernels.hpp"
+#include "lfft.hpp"
 
 namespace ps3d {
 
@@ -162,6 +163,15 @@ __device__ inline ZLds zlds(int QPB, int N, int nz) {
     return L;
 }
 
+// Batched complex FFT of the 2*QPB extension lines (length 2 nz) of a block.
+template <int NZ>
+__device__ inline double2 *zfft(const ZLds &L, int nlines, int N, const FftPlan &P, const double2 *tw) {
+    if constexpr (NZ == 0)
+        return fft_lds(L.A, L.B, nlines, N, P, tw, -1);
+    else
+        return cfft_lds<2 * NZ, CPlan8<2 * NZ>>(L.A, L.B, nlines, 2 * NZ, tw, -1);
+}
+
 // Broadcast the lid values (z = 0 and z = nz) of F to every thread of the quad.
 template <int ZPT>
 __device__ inline void get_lids(const ZT<ZPT> &T, const ZLds &L, const double (&F)[ZPT][4], double (&lo)[4],
@@ -184,7 +194,7 @@ __device__ inline void get_lids(const ZT<ZPT> &T, const ZLds &L, const double (&
 
 // Interior DST-I (stafft.f90:489-550, orthonormal, self-inverse) of the four
 // members of F: z = 1..nz-1 replaced, z = 0 and z = nz untouched.
-template <int ZPT>
+template <int ZPT, int NZ>
 __device__ inline void zdst4(const ZT<ZPT> &T, const ZLds &L, double (&F)[ZPT][4], const FftPlan &P,
                              const double2 *tw, double scale) {
     const int n = T.nz, N = T.N;
@@ -205,7 +215,7 @@ __device__ inline void zdst4(const ZT<ZPT> &T, const ZLds &L, double (&F)[ZPT][4
         }
     }
     __syncthreads();
-    double2 *R = fft_lds(L.A, L.B, T.QPB * 2, N, P, tw, -1);
+    double2 *R = zfft<NZ>(L, T.QPB * 2, N, P, tw);
     const double2 *r0 = R + (size_t)(T.qi * 2) * N;
     const double2 *r1 = r0 + N;
 #pragma unroll
@@ -223,7 +233,7 @@ __device__ inline void zdst4(const ZT<ZPT> &T, const ZLds &L, double (&F)[ZPT][4
 }
 
 // Full DCT-I (stafft.f90:410-483) of the four members of F (z = 0..nz).
-template <int ZPT>
+template <int ZPT, int NZ>
 __device__ inline void zdct4(const ZT<ZPT> &T, const ZLds &L, double (&F)[ZPT][4], const FftPlan &P,
                              const double2 *tw, double scale) {
     const int n = T.nz, N = T.N;
@@ -241,7 +251,7 @@ __device__ inline void zdct4(const ZT<ZPT> &T, const ZLds &L, double (&F)[ZPT][4
         }
     }
     __syncthreads();
-    double2 *R = fft_lds(L.A, L.B, T.QPB * 2, N, P, tw, -1);
+    double2 *R = zfft<NZ>(L, T.QPB * 2, N, P, tw);
     const double2 *r0 = R + (size_t)(T.qi * 2) * N;
     const double2 *r1 = r0 + N;
 #pragma unroll
@@ -286,7 +296,7 @@ __device__ inline void zdiff(const ZT<ZPT> &T, const ZLds &L, const Grid &g, con
 }
 
 // field_decompose_semi_spectral (inversion_utils.f90:563-592) in registers.
-template <int ZPT>
+template <int ZPT, int NZ>
 __device__ inline void zdecompose(const ZT<ZPT> &T, const ZLds &L, const Grid &g, double (&F)[ZPT][4],
                                   const FftPlan &P, const double2 *tw, double scale) {
     double lo[4], hi[4];
@@ -298,16 +308,16 @@ __device__ inline void zdecompose(const ZT<ZPT> &T, const ZLds &L, const Grid &g
         phi_only(g, T.K, T.q00, T.z[s], pm, pp);
         for (int m = 0; m < 4; ++m) F[s][m] = F[s][m] - (lo[m] * pm + hi[m] * pp);
     }
-    zdst4(T, L, F, P, tw, scale);
+    zdst4<ZPT, NZ>(T, L, F, P, tw, scale);
 }
 
 // field_combine_semi_spectral (inversion_utils.f90:617-647) in registers.
-template <int ZPT>
+template <int ZPT, int NZ>
 __device__ inline void zcombine(const ZT<ZPT> &T, const ZLds &L, const Grid &g, double (&F)[ZPT][4],
                                 const FftPlan &P, const double2 *tw, double scale) {
     double lo[4], hi[4];
     get_lids(T, L, F, lo, hi);
-    zdst4(T, L, F, P, tw, scale);
+    zdst4<ZPT, NZ>(T, L, F, P, tw, scale);
 #pragma unroll
     for (int s = 0; s < ZPT; ++s) {
         if (!T.has[s] || T.z[s] == 0 || T.z[s] == T.nz) continue;
@@ -360,8 +370,8 @@ __device__ inline void zstore(const ZT<ZPT> &T, double *__restrict__ dst, const 
 // solenoidally projected svor(xi, eta) and the z-midpoint vorticity
 // (omega(z-1) + omega(z))/2 at index z = 1..nz (for the characteristic vorticity).
 // ---------------------------------------------------------------------------
-template <int ZPT>
-__global__ void __launch_bounds__(256) k_zinv(ZInvArgs a) {
+template <int ZPT, int NZ>
+__global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     const Grid g = a.g;
     ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads);
     ZLds L = zlds(a.QPB, T.N, g.nz);
@@ -379,9 +389,9 @@ __global__ void __launch_bounds__(256) k_zinv(ZInvArgs a) {
     for (int s = 0; s < ZPT; ++s)
         for (int m = 0; m < 4; ++m) D[s][m] = T.fx[m] * Et[s][m ^ 1] - T.fy[m] * Xi[s][m ^ 2];
     // C = combine(zeta); E = decompose(central_diffz(C)) (46-48)
-    zcombine(T, L, g, C, P, a.twz, sc);
+    zcombine<ZPT, NZ>(T, L, g, C, P, a.twz, sc);
     zdiff(T, L, g, C, E);
-    zdecompose(T, L, g, E, P, a.twz, sc);
+    zdecompose<ZPT, NZ>(T, L, g, E, P, a.twz, sc);
 
     // projection A = k2l2i (E_x + D_y), B = k2l2i (E_y - D_x); (0,0) column kept (56-75)
     double Xp[ZPT][4], Ep[ZPT][4];
@@ -439,8 +449,8 @@ __global__ void __launch_bounds__(256) k_zinv(ZInvArgs a) {
             As[s][2] = (z > 0 && z < nz) ? rkzi * Xp[s][0] : 0.0;
         }
     }
-    zdct4(T, L, As, P, a.twz, sc);
-    zdst4(T, L, Ds, P, a.twz, sc);
+    zdct4<ZPT, NZ>(T, L, As, P, a.twz, sc);
+    zdst4<ZPT, NZ>(T, L, Ds, P, a.twz, sc);
 
     // complete w and dw/dz (134-141); xi, eta combined for the physical vorticity
 #pragma unroll
@@ -470,8 +480,8 @@ __global__ void __launch_bounds__(256) k_zinv(ZInvArgs a) {
             vbar[s] = As[s][2] - xln[0] * gamtop + xl0[0] * gambot;
         }
     }
-    zcombine(T, L, g, Xp, P, a.twz, sc);
-    zcombine(T, L, g, Ep, P, a.twz, sc);
+    zcombine<ZPT, NZ>(T, L, g, Xp, P, a.twz, sc);
+    zcombine<ZPT, NZ>(T, L, g, Ep, P, a.twz, sc);
 
     // u = k2l2i (es_x + cs_y), v = k2l2i (es_y - cs_x); (0,0): mean flow (169-213)
     double U[ZPT][4], V[ZPT][4];
@@ -592,8 +602,8 @@ __device__ inline void rk_update(const ZT<ZPT> &T, const RkArgs &rk, int mode, i
 // combine -> multiply by a (kx,ky) factor -> decompose sandwiches are applied as the
 // plain multiply they equal (both transforms are linear and act per column).
 // ---------------------------------------------------------------------------
-template <int ZPT>
-__global__ void __launch_bounds__(256) k_ztend(ZTendArgs a) {
+template <int ZPT, int NZ>
+__global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
     const Grid g = a.g;
     ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads);
     ZLds L = zlds(a.QPB, T.N, g.nz);
@@ -606,11 +616,11 @@ __global__ void __launch_bounds__(256) k_ztend(ZTendArgs a) {
     zload(T, a.rqp[2], Pp);
     zdiff(T, L, g, Q, FQ);
     zdiff(T, L, g, Pp, FP);
-    zdecompose(T, L, g, R, P, a.twz, sc);
-    zdecompose(T, L, g, Q, P, a.twz, sc);
-    zdecompose(T, L, g, Pp, P, a.twz, sc);
-    zdecompose(T, L, g, FQ, P, a.twz, sc);
-    zdecompose(T, L, g, FP, P, a.twz, sc);
+    zdecompose<ZPT, NZ>(T, L, g, R, P, a.twz, sc);
+    zdecompose<ZPT, NZ>(T, L, g, Q, P, a.twz, sc);
+    zdecompose<ZPT, NZ>(T, L, g, Pp, P, a.twz, sc);
+    zdecompose<ZPT, NZ>(T, L, g, FQ, P, a.twz, sc);
+    zdecompose<ZPT, NZ>(T, L, g, FP, P, a.twz, sc);
     // dxi/dt = r_y - q_z ; deta/dt = p_z - r_x ; dzeta/dt = q_x - p_y (341-367)
     double S[3][ZPT][4];
 #pragma unroll
@@ -630,7 +640,7 @@ __global__ void __launch_bounds__(256) k_ztend(ZTendArgs a) {
 
 // RK substep from a stored tendency (the API path: source then step).
 template <int ZPT>
-__global__ void __launch_bounds__(256) k_zrk(ZTendArgs a) {
+__global__ void __launch_bounds__(512) k_zrk(ZTendArgs a) {
     const Grid g = a.g;
     ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads);
     double S[3][ZPT][4];
@@ -640,8 +650,8 @@ __global__ void __launch_bounds__(256) k_zrk(ZTendArgs a) {
 
 // Single z operators on one volume (kernel checks): fftsine, fftcosine,
 // central_diffz, combine / decompose semi-spectral.
-template <int ZPT>
-__global__ void __launch_bounds__(256) k_zop(ZOpArgs a) {
+template <int ZPT, int NZ>
+__global__ void __launch_bounds__(512) k_zop(ZOpArgs a) {
     const Grid g = a.g;
     ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads);
     ZLds L = zlds(a.QPB, T.N, g.nz);
@@ -649,7 +659,7 @@ __global__ void __launch_bounds__(256) k_zop(ZOpArgs a) {
     zload(T, a.in, F);
     switch (a.op) {
         case ZOP_DST: {
-            zdst4(T, L, F, a.pz, a.twz, a.zscale);
+            zdst4<ZPT, NZ>(T, L, F, a.pz, a.twz, a.zscale);
             // stafft dst sets x(n) = 0 (stafft.f90:546-549); x(0) is not part of it
 #pragma unroll
             for (int s = 0; s < ZPT; ++s)
@@ -659,7 +669,7 @@ __global__ void __launch_bounds__(256) k_zop(ZOpArgs a) {
             break;
         }
         case ZOP_DCT:
-            zdct4(T, L, F, a.pz, a.twz, a.zscale);
+            zdct4<ZPT, NZ>(T, L, F, a.pz, a.twz, a.zscale);
             zstore(T, a.out, F);
             break;
         case ZOP_DIFFZ:
@@ -667,11 +677,11 @@ __global__ void __launch_bounds__(256) k_zop(ZOpArgs a) {
             zstore(T, a.out, O);
             break;
         case ZOP_COMBINE:
-            zcombine(T, L, g, F, a.pz, a.twz, a.zscale);
+            zcombine<ZPT, NZ>(T, L, g, F, a.pz, a.twz, a.zscale);
             zstore(T, a.out, F);
             break;
         case ZOP_DECOMPOSE:
-            zdecompose(T, L, g, F, a.pz, a.twz, a.zscale);
+            zdecompose<ZPT, NZ>(T, L, g, F, a.pz, a.twz, a.zscale);
             zstore(T, a.out, F);
             break;
         default:
@@ -750,46 +760,68 @@ size_t zpass_lds_bytes(int QPB, int nz) {
     return need + (size_t)QPB * 8 * sizeof(double);
 }
 
-#define ZDISPATCH(KERNEL, ARGS, LAUNCH)                                                                    \
-    switch (LAUNCH.zpt) {                                                                                  \
-        case 1: hipLaunchKernelGGL(KERNEL<1>, dim3(LAUNCH.blocks), dim3(256), LAUNCH.lds, stream, ARGS); break; \
-        case 2: hipLaunchKernelGGL(KERNEL<2>, dim3(LAUNCH.blocks), dim3(256), LAUNCH.lds, stream, ARGS); break; \
-        case 3: hipLaunchKernelGGL(KERNEL<3>, dim3(LAUNCH.blocks), dim3(256), LAUNCH.lds, stream, ARGS); break; \
-        case 5: hipLaunchKernelGGL(KERNEL<5>, dim3(LAUNCH.blocks), dim3(256), LAUNCH.lds, stream, ARGS); break; \
-        default: return hipErrorInvalidValue;                                                             \
-    }
+// (ZPT, nz) pairs with a compile-time FFT plan; other nz use the runtime plan.
+#define PS3D_Z_SIZES(X) X(1, 32) X(1, 48) X(1, 64) X(1, 96) X(1, 128) X(1, 192) X(1, 256) X(1, 384) \
+    X(2, 512) X(2, 768) X(3, 1024)
+
+#define ZLAUNCH(KERNEL, Z_, N_, ARGS, LAUNCH) \
+    hipLaunchKernelGGL((KERNEL<Z_, N_>), dim3(LAUNCH.blocks), dim3(LAUNCH.threads), LAUNCH.lds, stream, ARGS)
+
+#define ZDISPATCH_FFT(KERNEL, ARGS, LAUNCH)                                              \
+    do {                                                                                 \
+        bool done_ = false;                                                              \
+        _Pragma("clang diagnostic push")                                                 \
+        if (false) {}                                                                    \
+        PS3D_Z_SIZES(ZTRY_##KERNEL)                                                      \
+        _Pragma("clang diagnostic pop")                                                  \
+        if (!done_) {                                                                    \
+            switch (LAUNCH.zpt) {                                                        \
+                case 1: ZLAUNCH(KERNEL, 1, 0, ARGS, LAUNCH); break;                      \
+                case 2: ZLAUNCH(KERNEL, 2, 0, ARGS, LAUNCH); break;                      \
+                case 3: ZLAUNCH(KERNEL, 3, 0, ARGS, LAUNCH); break;                      \
+                default: return hipErrorInvalidValue;                                    \
+            }                                                                            \
+        }                                                                                \
+    } while (0)
+
+#define ZTRY_k_zinv(Z_, N_) else if (zl.zpt == Z_ && zl.nz == N_) { ZLAUNCH(k_zinv, Z_, N_, args, zl); done_ = true; }
+#define ZTRY_k_ztend(Z_, N_) else if (zl.zpt == Z_ && zl.nz == N_) { ZLAUNCH(k_ztend, Z_, N_, args, zl); done_ = true; }
+#define ZTRY_k_zop(Z_, N_) else if (zl.zpt == Z_ && zl.nz == N_) { ZLAUNCH(k_zop, Z_, N_, args, zl); done_ = true; }
 
 hipError_t configure_zpass() {
     const int lds = 160 * 1024;
     hipError_t e = hipSuccess;
-#define ZCFG(K)                                                                                        \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)K<1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)K<2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)K<3>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)K<5>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    ZCFG(k_zinv)
-    ZCFG(k_ztend)
-    ZCFG(k_zrk)
-    ZCFG(k_zop)
-#undef ZCFG
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_mean, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+#define SETA(K) \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)(K), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+#define ZCFG3(Z_, N_) SETA((k_zinv<Z_, N_>)) SETA((k_ztend<Z_, N_>)) SETA((k_zop<Z_, N_>))
+    PS3D_Z_SIZES(ZCFG3)
+    ZCFG3(1, 0) ZCFG3(2, 0) ZCFG3(3, 0)
+    SETA(k_zrk<1>) SETA(k_zrk<2>) SETA(k_zrk<3>)
+    SETA(k_mean)
+#undef ZCFG3
+#undef SETA
     return e;
 }
 
 hipError_t launch_zinv(const ZLaunch &zl, const ZInvArgs &args, hipStream_t stream) {
-    ZDISPATCH(k_zinv, args, zl);
+    ZDISPATCH_FFT(k_zinv, args, zl);
     return hipGetLastError();
 }
 hipError_t launch_ztend(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream) {
-    ZDISPATCH(k_ztend, args, zl);
+    ZDISPATCH_FFT(k_ztend, args, zl);
     return hipGetLastError();
 }
 hipError_t launch_zrk(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream) {
-    ZDISPATCH(k_zrk, args, zl);
+    switch (zl.zpt) {
+        case 1: hipLaunchKernelGGL(k_zrk<1>, dim3(zl.blocks), dim3(zl.threads), zl.lds, stream, args); break;
+        case 2: hipLaunchKernelGGL(k_zrk<2>, dim3(zl.blocks), dim3(zl.threads), zl.lds, stream, args); break;
+        case 3: hipLaunchKernelGGL(k_zrk<3>, dim3(zl.blocks), dim3(zl.threads), zl.lds, stream, args); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 hipError_t launch_zop(const ZLaunch &zl, const ZOpArgs &args, hipStream_t stream) {
-    ZDISPATCH(k_zop, args, zl);
+    ZDISPATCH_FFT(k_zop, args, zl);
     return hipGetLastError();
 }
 hipError_t launch_mean(const MeanArgs &args, int nz, hipStream_t stream) {
