@@ -147,8 +147,8 @@ __device__ inline Hyp hyp_all(const Grid &g, double K, double k2l2i, bool q00, i
 // LDS helpers
 // ---------------------------------------------------------------------------
 struct ZLds {
-    double2 *A, *B;    // FFT ping-pong: QPB*2 lines of N complex each
-    double *lid;       // QPB * 4 members * 2 (z=0, z=nz)
+    double2 *A, *B;    // FFT ping-pong: up to QPB*4 lines of N complex each
+    double *lid;       // QPB * 3 fields * 4 members * 2 (z=0, z=nz)
     double *colbuf;    // QPB * 4 * (nz+1)   (aliases A)
 };
 
@@ -156,8 +156,8 @@ __device__ inline ZLds zlds(int QPB, int N, int nz) {
     extern __shared__ double2 zsmem[];
     ZLds L;
     L.A = zsmem;
-    L.B = zsmem + (size_t)QPB * 2 * N;
-    L.lid = reinterpret_cast<double *>(L.B + (size_t)QPB * 2 * N);
+    L.B = zsmem + (size_t)QPB * 4 * N;
+    L.lid = reinterpret_cast<double *>(L.B + (size_t)QPB * 4 * N);
     L.colbuf = reinterpret_cast<double *>(L.A);
     (void)nz;
     return L;
@@ -176,7 +176,7 @@ __device__ inline double2 *zfft(const ZLds &L, int nlines, int N, const FftPlan 
 template <int ZPT>
 __device__ inline void get_lids(const ZT<ZPT> &T, const ZLds &L, const double (&F)[ZPT][4], double (&lo)[4],
                                 double (&hi)[4]) {
-    double *lid = L.lid + T.qi * 8;
+    double *lid = L.lid + T.qi * 24;
 #pragma unroll
     for (int s = 0; s < ZPT; ++s) {
         if (T.has[s] && T.z[s] == 0)
@@ -327,6 +327,176 @@ __device__ inline void zcombine(const ZT<ZPT> &T, const ZLds &L, const Grid &g, 
     }
 }
 
+
+enum { XK_DST = 0, XK_DCT = 1 };
+
+// Extension of the four members of F into two complex lines (DST-I: odd about
+// z = 0 and nz, interior only; DCT-I: even), and the read-back of the transform.
+template <int ZPT, int K>
+__device__ inline void ext_put(const ZT<ZPT> &T, double2 *l0, double2 *l1, const double (&F)[ZPT][4]) {
+    const int n = T.nz, N = T.N;
+#pragma unroll
+    for (int s = 0; s < ZPT; ++s) {
+        if (!T.has[s]) continue;
+        const int z = T.z[s];
+        if (K == XK_DST) {
+            if (z == 0 || z == n) {
+                l0[z] = make_double2(0.0, 0.0);
+                l1[z] = make_double2(0.0, 0.0);
+            } else {
+                l0[z] = make_double2(F[s][0], F[s][1]);
+                l0[N - z] = make_double2(-F[s][0], -F[s][1]);
+                l1[z] = make_double2(F[s][2], F[s][3]);
+                l1[N - z] = make_double2(-F[s][2], -F[s][3]);
+            }
+        } else {
+            l0[z] = make_double2(F[s][0], F[s][1]);
+            l1[z] = make_double2(F[s][2], F[s][3]);
+            if (z > 0 && z < n) {
+                l0[N - z] = l0[z];
+                l1[N - z] = l1[z];
+            }
+        }
+    }
+}
+template <int ZPT, int K>
+__device__ inline void ext_get(const ZT<ZPT> &T, const double2 *r0, const double2 *r1, double (&F)[ZPT][4],
+                               double scale) {
+    const int n = T.nz;
+#pragma unroll
+    for (int s = 0; s < ZPT; ++s) {
+        if (!T.has[s]) continue;
+        const int z = T.z[s];
+        const double2 a = r0[z], b = r1[z];
+        if (K == XK_DST) {
+            if (z == 0 || z == n) continue;
+            F[s][0] = -a.y * scale;
+            F[s][1] = a.x * scale;
+            F[s][2] = -b.y * scale;
+            F[s][3] = b.x * scale;
+        } else {
+            F[s][0] = a.x * scale;
+            F[s][1] = a.y * scale;
+            F[s][2] = b.x * scale;
+            F[s][3] = b.y * scale;
+        }
+    }
+}
+
+// Two z transforms of the quad in one batched FFT call (4 lines per quad).
+template <int ZPT, int NZ, int K0, int K1>
+__device__ inline void zxform2(const ZT<ZPT> &T, const ZLds &L, double (&F0)[ZPT][4], double (&F1)[ZPT][4],
+                               const FftPlan &P, const double2 *tw, double scale) {
+    const int N = T.N;
+    double2 *b = L.A + (size_t)(T.qi * 4) * N;
+    ext_put<ZPT, K0>(T, b, b + N, F0);
+    ext_put<ZPT, K1>(T, b + 2 * N, b + 3 * N, F1);
+    __syncthreads();
+    double2 *R = zfft<NZ>(L, T.QPB * 4, N, P, tw);
+    const double2 *r = R + (size_t)(T.qi * 4) * N;
+    ext_get<ZPT, K0>(T, r, r + N, F0, scale);
+    ext_get<ZPT, K1>(T, r + 2 * N, r + 3 * N, F1, scale);
+    __syncthreads();
+}
+
+// Lid values of three fields broadcast with one barrier pair.
+template <int ZPT>
+__device__ inline void get_lids3(const ZT<ZPT> &T, const ZLds &L, const double (&F0)[ZPT][4],
+                                 const double (&F1)[ZPT][4], const double (&F2)[ZPT][4], double (&lo)[3][4],
+                                 double (&hi)[3][4]) {
+    double *lid = L.lid + T.qi * 24;
+#pragma unroll
+    for (int s = 0; s < ZPT; ++s) {
+        if (!T.has[s]) continue;
+        const int off = T.z[s] == 0 ? 0 : (T.z[s] == T.nz ? 12 : -1);
+        if (off < 0) continue;
+        for (int m = 0; m < 4; ++m) {
+            lid[off + m] = F0[s][m];
+            lid[off + 4 + m] = F1[s][m];
+            lid[off + 8 + m] = F2[s][m];
+        }
+    }
+    __syncthreads();
+    for (int f = 0; f < 3; ++f)
+        for (int m = 0; m < 4; ++m) {
+            lo[f][m] = lid[4 * f + m];
+            hi[f][m] = lid[12 + 4 * f + m];
+        }
+    __syncthreads();
+}
+
+// central_diffz of one or two fields through one LDS column exchange; also returns
+// the lid values of each derivative (known to every thread from the columns).
+template <int ZPT>
+__device__ inline void fd_from_cols(const ZT<ZPT> &T, const Grid &g, const double *cb, double (&D)[ZPT][4],
+                                    double (&lo)[4], double (&hi)[4]) {
+    const int nz1 = T.nz + 1;
+#pragma unroll
+    for (int s = 0; s < ZPT; ++s) {
+        if (!T.has[s]) continue;
+        const int z = T.z[s];
+        for (int m = 0; m < 4; ++m) {
+            const double *c = cb + m * nz1;
+            if (z == 0)
+                D[s][m] = g.dzi * (c[1] - c[0]);
+            else if (z == T.nz)
+                D[s][m] = g.dzi * (c[T.nz] - c[T.nz - 1]);
+            else
+                D[s][m] = (c[z + 1] - c[z - 1]) * g.hdzi;
+        }
+    }
+    for (int m = 0; m < 4; ++m) {
+        const double *c = cb + m * nz1;
+        lo[m] = g.dzi * (c[1] - c[0]);
+        hi[m] = g.dzi * (c[T.nz] - c[T.nz - 1]);
+    }
+}
+template <int ZPT>
+__device__ inline void put_cols(const ZT<ZPT> &T, double *cb, const double (&F)[ZPT][4]) {
+    const int nz1 = T.nz + 1;
+#pragma unroll
+    for (int s = 0; s < ZPT; ++s)
+        if (T.has[s])
+            for (int m = 0; m < 4; ++m) cb[m * nz1 + T.z[s]] = F[s][m];
+}
+template <int ZPT>
+__device__ inline void zdiff2(const ZT<ZPT> &T, const ZLds &L, const Grid &g, const double (&F0)[ZPT][4],
+                              double (&D0)[ZPT][4], double (&lo0)[4], double (&hi0)[4], const double (&F1)[ZPT][4],
+                              double (&D1)[ZPT][4], double (&lo1)[4], double (&hi1)[4]) {
+    const int nz1 = T.nz + 1;
+    double *cb0 = L.colbuf + (size_t)T.qi * 12 * nz1;
+    double *cb1 = cb0 + 4 * nz1;
+    put_cols(T, cb0, F0);
+    put_cols(T, cb1, F1);
+    __syncthreads();
+    fd_from_cols(T, g, cb0, D0, lo0, hi0);
+    fd_from_cols(T, g, cb1, D1, lo1, hi1);
+    __syncthreads();
+}
+template <int ZPT>
+__device__ inline void zdiff1(const ZT<ZPT> &T, const ZLds &L, const Grid &g, const double (&F0)[ZPT][4],
+                              double (&D0)[ZPT][4], double (&lo0)[4], double (&hi0)[4]) {
+    const int nz1 = T.nz + 1;
+    double *cb0 = L.colbuf + (size_t)T.qi * 12 * nz1;
+    put_cols(T, cb0, F0);
+    __syncthreads();
+    fd_from_cols(T, g, cb0, D0, lo0, hi0);
+    __syncthreads();
+}
+
+// subtract (decompose) or add (combine) the harmonic interpolant of the lids
+template <int ZPT>
+__device__ inline void harmonic(const ZT<ZPT> &T, const Grid &g, double (&F)[ZPT][4], const double (&lo)[4],
+                                const double (&hi)[4], double sign) {
+#pragma unroll
+    for (int s = 0; s < ZPT; ++s) {
+        if (!T.has[s] || T.z[s] == 0 || T.z[s] == T.nz) continue;
+        double pm, pp;
+        phi_only(g, T.K, T.q00, T.z[s], pm, pp);
+        for (int m = 0; m < 4; ++m) F[s][m] = F[s][m] + sign * (lo[m] * pm + hi[m] * pp);
+    }
+}
+
 template <int ZPT>
 __device__ inline void zmid(const ZT<ZPT> &T, const ZLds &L, const double (&F)[ZPT][4], double *__restrict__ dst) {
     const int nz1 = T.nz + 1;
@@ -346,6 +516,34 @@ __device__ inline void zmid(const ZT<ZPT> &T, const ZLds &L, const double (&F)[Z
             dst[T.col[m] + z] = v;
         }
     }
+    __syncthreads();
+}
+
+template <int ZPT>
+__device__ inline void mid_store(const ZT<ZPT> &T, const double *cb, double *__restrict__ dst) {
+    const int nz1 = T.nz + 1;
+#pragma unroll
+    for (int s = 0; s < ZPT; ++s) {
+        const int z = T.z[s];
+        if (!T.has[s]) continue;
+        for (int m = 0; m < 4; ++m) {
+            if (!T.store[m]) continue;
+            dst[T.col[m] + z] = (z > 0) ? 0.5 * (cb[m * nz1 + z - 1] + cb[m * nz1 + z]) : 0.0;
+        }
+    }
+}
+template <int ZPT>
+__device__ inline void zmid3(const ZT<ZPT> &T, const ZLds &L, const double (&F0)[ZPT][4], const double (&F1)[ZPT][4],
+                             const double (&F2)[ZPT][4], double *d0, double *d1, double *d2) {
+    const int nz1 = T.nz + 1;
+    double *cb = L.colbuf + (size_t)T.qi * 12 * nz1;
+    put_cols(T, cb, F0);
+    put_cols(T, cb + 4 * nz1, F1);
+    put_cols(T, cb + 8 * nz1, F2);
+    __syncthreads();
+    mid_store(T, cb, d0);
+    mid_store(T, cb + 4 * nz1, d1);
+    mid_store(T, cb + 8 * nz1, d2);
     __syncthreads();
 }
 
@@ -389,9 +587,16 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     for (int s = 0; s < ZPT; ++s)
         for (int m = 0; m < 4; ++m) D[s][m] = T.fx[m] * Et[s][m ^ 1] - T.fy[m] * Xi[s][m ^ 2];
     // C = combine(zeta); E = decompose(central_diffz(C)) (46-48)
-    zcombine<ZPT, NZ>(T, L, g, C, P, a.twz, sc);
-    zdiff(T, L, g, C, E);
-    zdecompose<ZPT, NZ>(T, L, g, E, P, a.twz, sc);
+    {
+        double c0[4], cn[4];
+        get_lids(T, L, C, c0, cn);
+        zdst4<ZPT, NZ>(T, L, C, P, a.twz, sc);
+        harmonic(T, g, C, c0, cn, 1.0);
+        double e0[4], en[4];
+        zdiff1(T, L, g, C, E, e0, en);
+        harmonic(T, g, E, e0, en, -1.0);
+        zdst4<ZPT, NZ>(T, L, E, P, a.twz, sc);
+    }
 
     // projection A = k2l2i (E_x + D_y), B = k2l2i (E_y - D_x); (0,0) column kept (56-75)
     double Xp[ZPT][4], Ep[ZPT][4];
@@ -416,12 +621,12 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
         for (int m = 0; m < 4; ++m) Ds[s][m] = T.fy[m] * Xp[s][m ^ 2] - T.fx[m] * Ep[s][m ^ 1];
-    double d0[4], dn[4];
-    get_lids(T, L, Ds, d0, dn);
-    // lid values of the projected A, B for the mean flow of the (0,0) column
-    double xl0[4], xln[4], el0[4], eln[4];
-    get_lids(T, L, Xp, xl0, xln);
-    get_lids(T, L, Ep, el0, eln);
+    // lid values of ds (boundary part of w) and of the projected A, B (mean flow
+    // of the (0,0) column and the combine below)
+    double lo3[3][4], hi3[3][4];
+    get_lids3(T, L, Ds, Xp, Ep, lo3, hi3);
+    const double(&d0)[4] = lo3[0];
+    const double(&dn)[4] = hi3[0];
 
     // bs (interior) and es (all z): boundary parts (94-104); green and d/dz of the
     // sine part (107-122)
@@ -449,8 +654,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
             As[s][2] = (z > 0 && z < nz) ? rkzi * Xp[s][0] : 0.0;
         }
     }
-    zdct4<ZPT, NZ>(T, L, As, P, a.twz, sc);
-    zdst4<ZPT, NZ>(T, L, Ds, P, a.twz, sc);
+    zxform2<ZPT, NZ, XK_DCT, XK_DST>(T, L, As, Ds, P, a.twz, sc);
 
     // complete w and dw/dz (134-141); xi, eta combined for the physical vorticity
 #pragma unroll
@@ -476,12 +680,13 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
             const double pt = zp / g.lz, pb = zp2 / g.lz;
             const double gamtop = 0.5 * g.lz * (pt * pt - 1.0 / 3.0);
             const double gambot = 0.5 * g.lz * (pb * pb - 1.0 / 3.0);
-            ubar[s] = As[s][1] + eln[0] * gamtop - el0[0] * gambot;
-            vbar[s] = As[s][2] - xln[0] * gamtop + xl0[0] * gambot;
+            ubar[s] = As[s][1] + hi3[2][0] * gamtop - lo3[2][0] * gambot;
+            vbar[s] = As[s][2] - hi3[1][0] * gamtop + lo3[1][0] * gambot;
         }
     }
-    zcombine<ZPT, NZ>(T, L, g, Xp, P, a.twz, sc);
-    zcombine<ZPT, NZ>(T, L, g, Ep, P, a.twz, sc);
+    zxform2<ZPT, NZ, XK_DST, XK_DST>(T, L, Xp, Ep, P, a.twz, sc);
+    harmonic(T, g, Xp, lo3[1], hi3[1], 1.0);
+    harmonic(T, g, Ep, lo3[2], hi3[2], 1.0);
 
     // u = k2l2i (es_x + cs_y), v = k2l2i (es_y - cs_x); (0,0): mean flow (169-213)
     double U[ZPT][4], V[ZPT][4];
@@ -506,9 +711,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     if (a.mid[0]) {
         // (omega(z-1) + omega(z))/2 at z = 1..nz (0 at z = 0): the cell-midpoint
         // vorticity of get_char_vorticity (field_diagnostics.f90:509-521)
-        zmid(T, L, Xp, a.mid[0]);
-        zmid(T, L, Ep, a.mid[1]);
-        zmid(T, L, C, a.mid[2]);
+        zmid3(T, L, Xp, Ep, C, a.mid[0], a.mid[1], a.mid[2]);
     }
 }
 
@@ -614,13 +817,28 @@ __global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
     zload(T, a.rqp[0], R);
     zload(T, a.rqp[1], Q);
     zload(T, a.rqp[2], Pp);
-    zdiff(T, L, g, Q, FQ);
-    zdiff(T, L, g, Pp, FP);
-    zdecompose<ZPT, NZ>(T, L, g, R, P, a.twz, sc);
-    zdecompose<ZPT, NZ>(T, L, g, Q, P, a.twz, sc);
-    zdecompose<ZPT, NZ>(T, L, g, Pp, P, a.twz, sc);
-    zdecompose<ZPT, NZ>(T, L, g, FQ, P, a.twz, sc);
-    zdecompose<ZPT, NZ>(T, L, g, FP, P, a.twz, sc);
+    // central_diffz of q and p (342, 356) and the lids of all five fields
+    double fq0[4], fqn[4], fp0[4], fpn[4];
+    zdiff2(T, L, g, Q, FQ, fq0, fqn, Pp, FP, fp0, fpn);
+    double lo3[3][4], hi3[3][4];
+    get_lids3(T, L, R, Q, Pp, lo3, hi3);
+    // field_decompose of r, q, p, dq/dz, dp/dz: subtract the harmonic part, then DST
+#pragma unroll
+    for (int s = 0; s < ZPT; ++s) {
+        if (!T.has[s] || T.z[s] == 0 || T.z[s] == T.nz) continue;
+        double pm, pp;
+        phi_only(g, T.K, T.q00, T.z[s], pm, pp);
+        for (int m = 0; m < 4; ++m) {
+            R[s][m] = R[s][m] - (lo3[0][m] * pm + hi3[0][m] * pp);
+            Q[s][m] = Q[s][m] - (lo3[1][m] * pm + hi3[1][m] * pp);
+            Pp[s][m] = Pp[s][m] - (lo3[2][m] * pm + hi3[2][m] * pp);
+            FQ[s][m] = FQ[s][m] - (fq0[m] * pm + fqn[m] * pp);
+            FP[s][m] = FP[s][m] - (fp0[m] * pm + fpn[m] * pp);
+        }
+    }
+    zxform2<ZPT, NZ, XK_DST, XK_DST>(T, L, R, FQ, P, a.twz, sc);
+    zxform2<ZPT, NZ, XK_DST, XK_DST>(T, L, FP, Q, P, a.twz, sc);
+    zdst4<ZPT, NZ>(T, L, Pp, P, a.twz, sc);
     // dxi/dt = r_y - q_z ; deta/dt = p_z - r_x ; dzeta/dt = q_x - p_y (341-367)
     double S[3][ZPT][4];
 #pragma unroll
@@ -754,10 +972,10 @@ __global__ void __launch_bounds__(256) k_mean(MeanArgs a) {
 // ---------------------------------------------------------------------------
 size_t zpass_lds_bytes(int QPB, int nz) {
     const size_t N = 2 * (size_t)nz;
-    size_t fft = 2 * (size_t)QPB * 2 * N * sizeof(double2);
-    size_t col = (size_t)QPB * 4 * (nz + 1) * sizeof(double);
+    size_t fft = 2 * (size_t)QPB * 4 * N * sizeof(double2);
+    size_t col = (size_t)QPB * 12 * (nz + 1) * sizeof(double);
     size_t need = fft > col ? fft : col;
-    return need + (size_t)QPB * 8 * sizeof(double);
+    return need + (size_t)QPB * 24 * sizeof(double);
 }
 
 // (ZPT, nz) pairs with a compile-time FFT plan; other nz use the runtime plan.
