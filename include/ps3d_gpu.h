@@ -141,8 +141,12 @@ int ps3d_gpu_step(ps3d_handle *h, double *t, double dt);
  * diag_series (may be NULL) receives n records of PS3D_NDIAG doubles. */
 int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_series);
 
-/* = pressure + horizontal_divergence (src/fields/fields_derived.f90:67-182) from the
- * current vor2vel state; results are read with PS3D_PRES / PS3D_DELTA. */
+/* = pressure + horizontal_divergence (src/fields/fields_derived.f90:67-182), which
+ * ps3d's adapt calls (src/stepper/advance.f90:278-282).  Computed from the state of
+ * the last ps3d_gpu_vor2vel; call it after vor2vel (and adapt) and before source,
+ * where write_step sits in advance (advance.f90:85-91).  Neither field feeds the
+ * time step, so ps3d_gpu_advance_n leaves them out; results are read with
+ * PS3D_PRES / PS3D_DELTA. */
 int ps3d_gpu_derived(ps3d_handle *h);
 
 /* Device <-> host copies of a field (enum ps3d_field). */

@@ -26,7 +26,7 @@ enum Scal {
 
 // z-kernel modes
 enum { ZT_STORE = 0, ZT_RK4 = 1, ZT_CN2 = 2 };
-enum { ZOP_DST = 0, ZOP_DCT = 1, ZOP_DIFFZ = 2, ZOP_COMBINE = 3, ZOP_DECOMPOSE = 4 };
+enum { ZOP_DST = 0, ZOP_DCT = 1, ZOP_DIFFZ = 2, ZOP_COMBINE = 3, ZOP_DECOMPOSE = 4, ZOP_POISSON = 5 };
 enum { MEAN_CALC = 0, MEAN_ADJUST = 1 };
 
 // 1-D tables on the device
@@ -83,7 +83,7 @@ struct ZOpArgs {
     int TQ, QPB, nquads;
     FftPlan pz;
     const double2 *twz;
-    double zscale;
+    double zscale, rkz_sc;
     const double *in;
     double *out;
     int op;
@@ -105,6 +105,16 @@ hipError_t launch_ztend(const ZLaunch &zl, const ZTendArgs &args, hipStream_t st
 hipError_t launch_zrk(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream);
 hipError_t launch_zop(const ZLaunch &zl, const ZOpArgs &args, hipStream_t stream);
 hipError_t launch_mean(const MeanArgs &args, int nz, hipStream_t stream);
+
+// Pointwise kernels of the output-step derived fields (fields_derived.f90:67-182),
+// over whole padded volumes.
+struct PresArgs {
+    long n;                     // doubles per padded volume
+    const double *dudx, *dudy, *dvdy, *dwdx, *dwdy, *xi, *eta, *zeta;
+    double *pres;
+};
+hipError_t launch_pres_rhs(const PresArgs &a, hipStream_t stream);
+hipError_t launch_add2(long n, const double *a, const double *b, double *out, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // x / y line passes

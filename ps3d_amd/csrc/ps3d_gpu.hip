@@ -54,6 +54,7 @@ struct ps3d_handle {
     double *pool = nullptr;
     double *svor[3]{}, *qdi[3]{}, *qdf[3]{}, *svel[3]{}, *vel[3]{}, *vor[3]{}, *grad[3]{}, *mid[3]{};
     double *scal = nullptr, *party = nullptr, *partx = nullptr, *roll = nullptr;
+    double *pres = nullptr, *delta = nullptr;   // output-step derived fields, allocated on first use
     double *diag = nullptr;
     int diag_cap = 0;
     double *rkx = nullptr, *rky = nullptr, *filx = nullptr, *fily = nullptr, *filz = nullptr;
@@ -64,6 +65,8 @@ struct ps3d_handle {
     LineCfg lx{}, ly{};
     double visc = 0.0;
     bool have_state = false, have_phys = false, have_adapt = false;
+    bool have_grad = false;   // grad holds the physical y-derivatives of u, v, w (vor2vel, before source)
+    bool have_derived = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_steps = 0;
@@ -179,6 +182,8 @@ static void free_handle(ps3d_handle *h) {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
     hipFree(h->pool);
+    hipFree(h->pres);
+    hipFree(h->delta);
     hipFree(h->scal);
     hipFree(h->party);
     hipFree(h->partx);
@@ -436,6 +441,7 @@ static ZOpArgs zop_args(ps3d_handle *h, int op, const double *in, double *out) {
     a.pz = h->pz;
     a.twz = h->twz;
     a.zscale = 1.0 / std::sqrt(2.0 * h->g.nz);
+    a.rkz_sc = 3.141592653589793238462643383279502884 / h->p.extent[2];
     a.in = in;
     a.out = out;
     a.op = op;
@@ -705,6 +711,7 @@ int ps3d_gpu_vor2vel(ps3d_handle *h) {
     }
     if ((st = line_fields(h, 0, LM_INV, 12, f, f))) return st;
     h->have_phys = true;
+    h->have_grad = true;
     h->have_adapt = false;
     return 0;
 }
@@ -732,6 +739,7 @@ int ps3d_gpu_source(ps3d_handle *h) {
     if (!h->have_phys) return fail(PS3D_E_STATE, "source needs a preceding vor2vel");
     int st;
     // r, q, p into grad (adapt has consumed it), then the z part into svorts (= mid)
+    h->have_grad = false;
     if ((st = products(h, 1, h->grad))) return st;
     ZTendArgs t = ztend_args(h, ZT_STORE, 0, h->grad);
     HIPCHK(launch_ztend(h->zl, t, h->stream));
@@ -763,6 +771,7 @@ int ps3d_gpu_step(ps3d_handle *h, double *t, double dt) {
     *t = sc[SC_T];
     h->have_phys = false;
     h->have_adapt = false;
+    h->have_grad = false;
     return 0;
 }
 
@@ -790,6 +799,7 @@ int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_serie
         HIPCHK(hipMemcpy(diag_series, h->diag, sizeof(double) * PS3D_NDIAG * (size_t)nsteps, hipMemcpyDeviceToHost));
     h->have_phys = false;
     h->have_adapt = false;
+    h->have_grad = false;
     return 0;
 }
 
@@ -860,9 +870,51 @@ int ps3d_gpu_last_timing(ps3d_handle *h, double *ms, int *nsteps) {
     return 0;
 }
 
+// pressure + horizontal_divergence (fields_derived.f90:67-182) from the state of the
+// last vor2vel: svel (semi-spectral), vor and the y-derivatives du/dy, dv/dy, dw/dy
+// (physical).  qdi / qdf serve as scratch: they only carry data inside a step.
 int ps3d_gpu_derived(ps3d_handle *h) {
-    (void)h;
-    return fail(PS3D_E_UNSUPPORTED, "pressure / horizontal divergence not available yet");
+    if (!h) return fail(PS3D_E_ARG, "null handle");
+    if (!h->have_phys || !h->have_grad)
+        return fail(PS3D_E_STATE, "derived needs a preceding vor2vel (and no source since)");
+    const size_t bytes = sizeof(double) * h->nvol;
+    if (!h->pres) HIPCHK(hipMalloc((void **)&h->pres, bytes));
+    if (!h->delta) HIPCHK(hipMalloc((void **)&h->delta, bytes));
+    int st;
+    double *dudx = h->qdi[0], *dwdx = h->qdi[1], *dvdy_s = h->qdi[2], *div_s = h->qdf[0];
+    {   // du/dx, dw/dx (spectral) and dv/dy (spectral)
+        double *in[2] = {h->svel[0], h->svel[2]};
+        double *out[2] = {dudx, dwdx};
+        if ((st = line_fields(h, 0, LM_DIFF, 2, in, out))) return st;
+        double *in1[1] = {h->svel[1]};
+        double *out1[1] = {dvdy_s};
+        if ((st = line_fields(h, 1, LM_DIFF, 1, in1, out1))) return st;
+    }
+    HIPCHK(launch_add2((long)h->nvol, dudx, dvdy_s, div_s, h->stream));
+    {   // fftxys2p of du/dx, dw/dx and the divergence
+        double *in[3] = {dudx, dwdx, div_s};
+        double *out[3] = {dudx, dwdx, h->delta};
+        if ((st = line_fields(h, 0, LM_INV, 3, in, out))) return st;
+        if ((st = line_fields(h, 1, LM_INV, 3, out, out))) return st;
+    }
+    PresArgs pa;
+    pa.n = (long)h->nvol;
+    pa.dudx = dudx;
+    pa.dudy = h->grad[0];
+    pa.dvdy = h->grad[1];
+    pa.dwdx = dwdx;
+    pa.dwdy = h->grad[2];
+    pa.xi = h->vor[0];
+    pa.eta = h->vor[1];
+    pa.zeta = h->vor[2];
+    pa.pres = h->pres;
+    HIPCHK(launch_pres_rhs(pa, h->stream));
+    double *p1[1] = {h->pres};
+    if ((st = line_fields(h, 1, LM_FWD, 1, p1, p1)) || (st = line_fields(h, 0, LM_FWD, 1, p1, p1))) return st;
+    HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_POISSON, h->pres, h->pres), h->stream));
+    if ((st = line_fields(h, 0, LM_INV, 1, p1, p1)) || (st = line_fields(h, 1, LM_INV, 1, p1, p1))) return st;
+    h->have_derived = true;
+    return 0;
 }
 
 int ps3d_gpu_get(ps3d_handle *h, int which, double *host) {
@@ -875,10 +927,16 @@ int ps3d_gpu_get(ps3d_handle *h, int which, double *host) {
         case PS3D_VEL: src = h->vel; break;
         case PS3D_SVEL: src = h->svel; break;
         case PS3D_SVORTS: src = h->mid; break;
+        case PS3D_PRES:
+        case PS3D_DELTA:
+            if (!h->have_derived) return fail(PS3D_E_STATE, "pressure / divergence need ps3d_gpu_derived first");
+            src = which == PS3D_PRES ? &h->pres : &h->delta;
+            break;
         default: return fail(PS3D_E_UNSUPPORTED, "field not available");
     }
+    const int nc = (which == PS3D_PRES || which == PS3D_DELTA) ? 1 : 3;
     int st;
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < nc; ++c)
         if ((st = copy_out(h, host + c * hv, src[c]))) return st;
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
@@ -903,6 +961,7 @@ int ps3d_gpu_set(ps3d_handle *h, int which, const double *host) {
 int ps3d_gpu_op(ps3d_handle *h, int op, const double *in, double *out) {
     if (!h || !in || !out) return fail(PS3D_E_ARG, "null argument");
     double *A = h->grad[0], *B = h->grad[1];
+    h->have_grad = false;
     int st;
     if ((st = copy_in(h, A, in))) return st;
     double *a1[1] = {A};

@@ -11,6 +11,8 @@
 // registers; z transforms (DST-I of the interior, DCT-I of the whole column,
 // src/fft/stafft.f90:410-550) go through LDS as complex FFTs of length 2nz of the
 // odd/even extension, two columns per complex line.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "lfft.hpp"
 
@@ -902,9 +904,58 @@ __global__ void __launch_bounds__(512) k_zop(ZOpArgs a) {
             zdecompose<ZPT, NZ>(T, L, g, F, a.pz, a.twz, a.zscale);
             zstore(T, a.out, F);
             break;
+        case ZOP_POISSON: {
+            // pressure's z part (fields_derived.f90:117-137): dct, times green, dct.
+            // green = -1/(k2l2 + rkz^2) for kz >= 1 and -k2l2i at kz = 0
+            // (inversion_utils.f90:240-258); k2l2 is the same for the 4 quad members.
+            zdct4<ZPT, NZ>(T, L, F, a.pz, a.twz, a.zscale);
+#pragma unroll
+            for (int s = 0; s < ZPT; ++s) {
+                const int z = T.z[s];
+                const double rkz = a.rkz_sc * (double)z;
+                const double green = z == 0 ? -T.k2l2i : -1.0 / (T.k2l2 + rkz * rkz);
+                for (int m = 0; m < 4; ++m) F[s][m] = green * F[s][m];
+            }
+            zdct4<ZPT, NZ>(T, L, F, a.pz, a.twz, a.zscale);
+            zstore(T, a.out, F);
+            break;
+        }
         default:
             break;
     }
+}
+
+// Right-hand side of the pressure Poisson equation, 2 (J_xy(u,v) + J_yz(v,w) +
+// J_zx(w,u)) with dv/dx = zeta + du/dy, du/dz = eta + dw/dx, dv/dz = dw/dy - xi,
+// dw/dz = -(du/dx + dv/dy) (fields_derived.f90:89-104), same operation order.
+__global__ void __launch_bounds__(256) k_pres_rhs(PresArgs a) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
+        const double dudx = a.dudx[i], dudy = a.dudy[i], dvdy = a.dvdy[i], dwdx = a.dwdx[i], dwdy = a.dwdy[i];
+        const double dwdz = -(dudx + dvdy);
+        a.pres[i] = 2.0 * (dudx * dvdy - dudy * (a.zeta[i] + dudy) + dvdy * dwdz - dwdy * (dwdy - a.xi[i]) +
+                           dwdz * dudx - dwdx * (dwdx + a.eta[i]));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_add2(long n, const double *__restrict__ x, const double *__restrict__ y,
+                                               double *__restrict__ out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = x[i] + y[i];
+}
+
+static unsigned pointwise_blocks(long n) {
+    const long b = (n + 255) / 256;
+    return (unsigned)std::min<long>(b, 256L * 64L);
+}
+
+hipError_t launch_pres_rhs(const PresArgs &a, hipStream_t stream) {
+    hipLaunchKernelGGL(k_pres_rhs, dim3(pointwise_blocks(a.n)), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_add2(long n, const double *a, const double *b, double *out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_add2, dim3(pointwise_blocks(n)), dim3(256), 0, stream, n, a, b, out);
+    return hipGetLastError();
 }
 
 // calc_vorticity_mean / adjust_vorticity_mean (field_diagnostics.f90:584-619) on

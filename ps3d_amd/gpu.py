@@ -127,6 +127,11 @@ class Ps3dGpu:
     def vor2vel(self):
         _check(lib().ps3d_gpu_vor2vel(self.h))
 
+    def derived(self):
+        """pressure + horizontal_divergence (fields_derived.f90:67-182) from the last
+        vor2vel; read the results with get(PRES) / get(DELTA)."""
+        _check(lib().ps3d_gpu_derived(self.h))
+
     def source(self):
         _check(lib().ps3d_gpu_source(self.h))
 

@@ -172,3 +172,31 @@ def test_fused_advance_equals_api_path(G):
     assert field_rel(a.get(G.SVOR), b.get(G.SVOR)) < 1e-13
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("n,nsteps", [(32, 0), (32, 3), (48, 2)])
+def test_derived_pressure_divergence(oracle_mod, G, n, nsteps):
+    """pressure + horizontal_divergence (fields_derived.f90:67-182) from the
+    vor2vel state after nsteps steps, against the oracle."""
+    o, d = make_pair(oracle_mod, G, n, **config1())
+    vor = ics.beltrami_multimode(n, n, n, seed=2)
+    o.set_vorticity(vor)
+    d.set_vorticity(vor)
+    if nsteps:
+        o.advance_n(nsteps)
+        d.advance_n(nsteps)
+    print("derived", n, nsteps, flush=True)
+    o.vor2vel()
+    d.vor2vel()
+    o.derived()
+    d.derived()
+    for which in (G.PRES, G.DELTA):
+        a, b = d.get(which), o.get(which)
+        r = np.abs(a - b).max() / np.abs(b).max()
+        print("field", which, r, flush=True)
+        assert r < TOL, which
+    # source consumes the derivative fields: derived must refuse afterwards
+    d.source()
+    with pytest.raises(G.Ps3dError):
+        d.derived()
+    d.close()
