@@ -42,6 +42,8 @@ struct ZLaunch {
     size_t lds;
     int nz;
     int threads;   // TQ * QPB
+    int wave;      // 1: one wavefront per column (zwave.hip), 0: quad-per-thread (zpass.hip)
+    int nquads;
 };
 
 struct ZInvArgs {
@@ -104,6 +106,14 @@ hipError_t launch_zinv(const ZLaunch &zl, const ZInvArgs &args, hipStream_t stre
 hipError_t launch_ztend(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream);
 hipError_t launch_zrk(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream);
 hipError_t launch_zop(const ZLaunch &zl, const ZOpArgs &args, hipStream_t stream);
+
+// wave-per-column z kernels (zwave.hip), nz in {128, 256, 512}
+bool zwave_supported(int nz);
+size_t zwave_lds_bytes(int nz);
+hipError_t configure_zwave();
+hipError_t launch_zinv_w(int nz, int nquads, const ZInvArgs &args, hipStream_t stream);
+hipError_t launch_ztend_w(int nz, int nquads, const ZTendArgs &args, hipStream_t stream);
+hipError_t launch_zop_w(int nz, int nquads, const ZOpArgs &args, hipStream_t stream);
 hipError_t launch_mean(const MeanArgs &args, int nz, hipStream_t stream);
 
 // Pointwise kernels of the output-step derived fields (fields_derived.f90:67-182),

@@ -293,11 +293,20 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     h->zl.lds = zpass_lds_bytes(h->QPB, nz);
     h->zl.nz = nz;
     if (h->zl.lds > 160 * 1024) return fail(PS3D_E_UNSUPPORTED, "nz too large for the LDS z transform");
+    // one wavefront per column where a wave FFT exists for 2 nz (PS3D_ZWAVE=0 keeps
+    // the quad-per-thread kernels, for comparison)
+    h->zl.nquads = h->nquads;
+    {
+        const char *ev = std::getenv("PS3D_ZWAVE");
+        const bool off = ev && ev[0] == '0';
+        h->zl.wave = (ps3d::zwave_supported(nz) && !off) ? 1 : 0;
+    }
 
     h->ly = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs);
     h->lx = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs);
 
     HIPCHK(ps3d::configure_zpass());
+    HIPCHK(ps3d::configure_zwave());
     HIPCHK(ps3d::configure_linepass());
 
     HIPCHK(hipMalloc((void **)&h->scal, sizeof(double) * SC_NSCAL));

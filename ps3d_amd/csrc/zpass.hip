@@ -15,6 +15,7 @@
 
 #include "kernels.hpp"
 #include "lfft.hpp"
+#include "zmath.hpp"
 
 namespace ps3d {
 
@@ -86,63 +87,6 @@ __device__ inline ZT<ZPT> make_zt(const Grid &g, const ZTabs &tb, int TQ, int QP
     T.k2l2i = T.q00 ? 0.0 : 1.0 / T.k2l2;
     T.K = sqrt(T.k2l2);
     return T;
-}
-
-// Hyperbolic functions of a quad at level iz (inversion_utils.f90:484-542 and the
-// (0,0) column at 326-346).  All four members share K = sqrt(k^2+l^2).
-struct Hyp {
-    double phim, phip, thetam, thetap, dthetam, dthetap;
-};
-
-__device__ inline void zm_zp(const Grid &g, int iz, double &zm, double &zp) {
-    // z = lower + dz*iz ; zm = upper - z ; zp = z - lower (inversion_utils.f90:291-297)
-    const double z = g.zlo + g.dz * (double)iz;
-    zm = (g.zlo + g.lz) - z;
-    zp = z - g.zlo;
-}
-
-__device__ inline void phi_only(const Grid &g, double K, bool q00, int iz, double &pm, double &pp) {
-    double zm, zp;
-    zm_zp(g, iz, zm, zp);
-    if (q00) {
-        pm = zm / g.lz;
-        pp = zp / g.lz;
-        return;
-    }
-    const double ef = exp(-K * g.lz);
-    const double div = 1.0 / (1.0 - ef * ef);
-    const double ep = exp(-K * zp), em = exp(-K * zm);
-    pm = div * (ep - ef * em);
-    pp = div * (em - ef * ep);
-}
-
-__device__ inline Hyp hyp_all(const Grid &g, double K, double k2l2i, bool q00, int iz) {
-    Hyp h;
-    double zm, zp;
-    zm_zp(g, iz, zm, zp);
-    if (q00) {
-        h.phim = zm / g.lz;
-        h.phip = zp / g.lz;
-        h.thetam = h.thetap = h.dthetam = h.dthetap = 0.0;
-        return h;
-    }
-    const double kl = K;
-    const double ef = exp(-kl * g.lz);
-    const double div = 1.0 / (1.0 - ef * ef);
-    const double k2ifac = 0.5 * k2l2i;
-    const double Lm = kl * zm, Lp = kl * zp;
-    const double ep = exp(-Lp), em = exp(-Lm);
-    h.phim = div * (ep - ef * em);
-    h.phip = div * (em - ef * ep);
-    const double dphim = -kl * div * (ep + ef * em);
-    const double dphip = kl * div * (em + ef * ep);
-    const double Q = div * (1.0 + ef * ef);
-    const double R = div * 2.0 * ef;
-    h.thetam = k2ifac * (R * Lm * h.phip - Q * Lp * h.phim);
-    h.thetap = k2ifac * (R * Lp * h.phim - Q * Lm * h.phip);
-    h.dthetam = -k2ifac * ((Q * Lp - 1.0) * dphim - R * Lm * dphip);
-    h.dthetap = -k2ifac * ((Q * Lm - 1.0) * dphip - R * Lp * dphim);
-    return h;
 }
 
 // ---------------------------------------------------------------------------
@@ -1073,10 +1017,12 @@ hipError_t configure_zpass() {
 }
 
 hipError_t launch_zinv(const ZLaunch &zl, const ZInvArgs &args, hipStream_t stream) {
+    if (zl.wave) return launch_zinv_w(zl.nz, zl.nquads, args, stream);
     ZDISPATCH_FFT(k_zinv, args, zl);
     return hipGetLastError();
 }
 hipError_t launch_ztend(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream) {
+    if (zl.wave) return launch_ztend_w(zl.nz, zl.nquads, args, stream);
     ZDISPATCH_FFT(k_ztend, args, zl);
     return hipGetLastError();
 }
@@ -1090,6 +1036,7 @@ hipError_t launch_zrk(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stre
     return hipGetLastError();
 }
 hipError_t launch_zop(const ZLaunch &zl, const ZOpArgs &args, hipStream_t stream) {
+    if (zl.wave) return launch_zop_w(zl.nz, zl.nquads, args, stream);
     ZDISPATCH_FFT(k_zop, args, zl);
     return hipGetLastError();
 }

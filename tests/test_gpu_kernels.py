@@ -27,7 +27,7 @@ def pair(oracle_mod, G, g, **kw):
     return oracle_mod.Oracle(po), G.Ps3dGpu(pg)
 
 
-@pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128, 256])
+@pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128, 256, 512])
 @pytest.mark.parametrize("m", [1, 6, 17])
 def test_fft1d_matches_oracle(oracle_mod, G, n, m):
     print("fft1d n=%d m=%d" % (n, m), flush=True)
@@ -45,6 +45,10 @@ GRIDS = [
     dict(nx=32, ny=32, nz=32, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI)),
     dict(nx=48, ny=16, nz=24, lower=(-1.0, 0.5, -0.3), extent=(3.0, 1.0, 1.7)),
     dict(nx=64, ny=96, nz=64, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 3 * PI, PI)),
+    # long columns: the wave-per-column z kernels (nz = 128, 256, 512)
+    dict(nx=16, ny=12, nz=128, lower=(0.0, 0.0, -0.5), extent=(2 * PI, 1.5, 2.0)),
+    dict(nx=8, ny=16, nz=256, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI)),
+    dict(nx=4, ny=6, nz=512, lower=(0.0, 0.0, 0.0), extent=(1.0, 2.0, 3.0)),
 ]
 
 

@@ -200,3 +200,46 @@ def test_derived_pressure_divergence(oracle_mod, G, n, nsteps):
     with pytest.raises(G.Ps3dError):
         d.derived()
     d.close()
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 128), (8, 16, 256), (4, 6, 512)])
+@pytest.mark.parametrize("filtering", ["Hou & Li", "2/3-rule"])
+def test_long_column_trajectory(oracle_mod, G, shape, filtering):
+    """Adaptive steps on grids whose z columns take the wave-per-column
+    kernels, against the oracle: fields and diagnostics within 1e-12."""
+    nx, ny, nz = shape
+    kw = dict(filtering=filtering, stepper="impl-diff-rk4", nnu=3, prediss=30.0, pretype="vorch",
+              length_scale="Kolmogorov", alpha=0.1, dt_fixed=0.0, limit=1.0e30)
+    g = dict(nx=nx, ny=ny, nz=nz, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI))
+    o = oracle_mod.Oracle(oracle_mod.make_params(**g, **kw))
+    d = G.Ps3dGpu(G.make_params(**g, **kw))
+    vor = ics.beltrami_multimode(nx, ny, nz, seed=3)
+    o.set_vorticity(vor)
+    d.set_vorticity(vor)
+    print("long column", shape, filtering, flush=True)
+    do = o.advance_n(8)
+    dd = d.advance_n(8)
+    r = diag_rel(dd, do, DIAG_COLS)
+    print("diag rel", max(r.values()), flush=True)
+    assert max(r.values()) < TOL, r
+    assert field_rel(d.get(G.SVOR), o.get(oracle_mod.SVOR)) < TOL
+    d.close()
+    o.close()
+
+
+def test_wave_and_quad_z_kernels_agree(G, monkeypatch):
+    """The wave-per-column and quad-per-thread z kernels give the same step."""
+    p = G.make_params(**box(32), **config1())
+    p.nz = 128
+    vor = ics.beltrami_multimode(32, 32, 128, seed=5)
+    monkeypatch.setenv("PS3D_ZWAVE", "0")
+    a = G.Ps3dGpu(p)
+    monkeypatch.setenv("PS3D_ZWAVE", "1")
+    b = G.Ps3dGpu(p)
+    a.set_vorticity(vor)
+    b.set_vorticity(vor)
+    a.advance_n(3)
+    b.advance_n(3)
+    assert field_rel(a.get(G.SVOR), b.get(G.SVOR)) < 1e-13
+    a.close()
+    b.close()
