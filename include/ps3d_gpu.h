@@ -198,6 +198,43 @@ int ps3d_gpu_sync(ps3d_handle *h);
  * handle's stream: total milliseconds and number of steps. */
 int ps3d_gpu_last_timing(ps3d_handle *h, double *ms, int *nsteps);
 
+/* ---- Batched instances and ensembles ------------------------------------------
+ * ps3d runs one grid per process: an ensemble there is many runs of the time loop
+ * (src/ps3d.f90:114-116), each calling advance (src/stepper/advance.f90:77-104).
+ * Here one batched handle holds nbatch instances of one grid and every stepping
+ * kernel covers all of them in one launch (the instance is the second grid
+ * dimension); an ensemble holds one batched handle per grid size, steps the
+ * groups concurrently on forked streams and can replay the whole step as one
+ * captured HIP graph.  The plain entry points above refuse batched handles. */
+
+/* nbatch instances of the grid p. */
+int ps3d_gpu_create_batch(const ps3d_params *p, int nbatch, ps3d_handle **h);
+int ps3d_gpu_batch_size(ps3d_handle *h, int *nbatch);
+/* setup_fields of instance b (as ps3d_gpu_set_vorticity). */
+int ps3d_gpu_set_vorticity_batch(ps3d_handle *h, int b, const double *vor);
+/* svor of instance b (PS3D_SVOR only). */
+int ps3d_gpu_get_batch(ps3d_handle *h, int b, int which, double *host);
+/* nsteps of advance for every instance; t[b] in/out per instance.  diag_series
+ * (may be NULL) receives nbatch x nsteps records of PS3D_NDIAG doubles. */
+int ps3d_gpu_advance_n_batch(ps3d_handle *h, double *t, int nsteps, double *diag_series);
+
+typedef struct ps3d_ensemble ps3d_ensemble;
+/* ngroups batched handles: group g is the grid p[g] with nbatch[g] instances. */
+int ps3d_gpu_ensemble_create(int ngroups, const ps3d_params *p, const int *nbatch, ps3d_ensemble **e);
+/* The batched handle of group g (owned by the ensemble; set initial vorticity
+ * with ps3d_gpu_set_vorticity_batch, read with ps3d_gpu_get_batch). */
+int ps3d_gpu_ensemble_group(ps3d_ensemble *e, int g, ps3d_handle **h);
+/* nsteps steps of every instance of every group; use_graph = 1 replays one
+ * captured graph per step.  ms: kernel time (HIP events), may be NULL. */
+int ps3d_gpu_ensemble_advance(ps3d_ensemble *e, int nsteps, int use_graph, double *ms);
+/* Times `steps` steps of each group under each available z launch shape and
+ * keeps the fastest (state restored afterwards).  choice[g] (may be NULL)
+ * receives the shape code: 100 = one wavefront per column, q < 100 = q quads per
+ * block of the quad-per-thread kernels. */
+int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice);
+int ps3d_gpu_ensemble_set_choice(ps3d_ensemble *e, int g, int code);
+int ps3d_gpu_ensemble_destroy(ps3d_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,15 @@ enum { ZT_STORE = 0, ZT_RK4 = 1, ZT_CN2 = 2 };
 enum { ZOP_DST = 0, ZOP_DCT = 1, ZOP_DIFFZ = 2, ZOP_COMBINE = 3, ZOP_DECOMPOSE = 4, ZOP_POISSON = 5 };
 enum { MEAN_CALC = 0, MEAN_ADJUST = 1 };
 
+// Batched instances of one grid (an ensemble group): blockIdx.y selects the
+// instance, whose data sit at these element strides from instance 0.
+struct Batch {
+    int nb;                          // instances (gridDim.y); 0 or 1: a single grid
+    long vol;                        // field pool
+    long part;                       // per-block partials of a line pass
+    long party, partx, diag, roll;   // finishing reductions
+};
+
 // 1-D tables on the device
 struct ZTabs {
     const double *rkx, *rky;      // wavenumbers (sta3dfft.f90:86-104)
@@ -58,6 +67,7 @@ struct ZInvArgs {
     double *svor_out[2];
     double *out[6];
     double *mid[3];
+    Batch bt;
 };
 
 struct RkArgs {
@@ -77,6 +87,7 @@ struct ZTendArgs {
     double *svorts[3];
     int mode, stage;
     RkArgs rk;
+    Batch bt;
 };
 
 struct ZOpArgs {
@@ -99,6 +110,7 @@ struct MeanArgs {
     double *svor[2];
     double *scal;
     int mode, advance_time;
+    Batch bt;
 };
 
 size_t zpass_lds_bytes(int QPB, int nz);
@@ -163,6 +175,7 @@ struct LineArgs {
     double fcor[3];
     double ncelli;
     double sqrt_nx;    // for means from the kx = 0 slot
+    Batch bt;
 };
 
 size_t line_lds_bytes(int n, int ZC);
@@ -182,6 +195,7 @@ struct FinArgs {
     int pretype, stepper, nnu;
     int roll_len;
     double *roll_hist;
+    Batch bt;
 };
 hipError_t launch_fin_y(const FinArgs &a, hipStream_t stream);
 hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream);

@@ -48,7 +48,7 @@ __device__ inline Tile make_tile(const LineArgs &a) {
     t.z0 = zt * a.ZC;
     const int rem = a.g.nz + 1 - t.z0;
     t.zce = rem < a.ZC ? rem : a.ZC;
-    t.base = (long)t.other * a.ostride + t.z0;
+    t.base = (long)blockIdx.y * a.bt.vol + (long)t.other * a.ostride + t.z0;
     t.twl = nullptr;
     return t;
 }
@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         m1 = block_sum(m1, scratch);
         m2 = block_sum(m2, scratch);
         if (threadIdx.x == 0) {
-            double *p = a.part + (size_t)blockIdx.x * RY_N;
+            double *p = a.part + (size_t)blockIdx.y * a.bt.part + (size_t)blockIdx.x * RY_N;
             p[RY_UU] = suu;
             p[RY_WW] = sww;
             p[RY_UW] = suw;
@@ -488,7 +488,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         // adapt (advance.f90:171-289): max |omega|, signed max of u, v, w, largest
         // |eigenvalue| of the symmetrised strain (Jacobi per point), lid maxima, and
         // the sums of the characteristic vorticity.
-        const double vortrms = a.scal[SC_VORTRMS];
+        const double vortrms = a.scal[(size_t)blockIdx.y * SC_NSCAL + SC_VORTRMS];
         double umax = -INFINITY, vmax = -INFINITY, wmax = -INFINITY, w2max = 0.0;
         double gg = 2.220446049250313e-16, usgg = 0.0, lsgg = 0.0, l1 = 0.0, l2 = 0.0;
         double2 ux[kEpt], uy[kEpt], vy[kEpt], wx[kEpt], wy[kEpt], xi[kEpt], et[kEpt], ze[kEpt];
@@ -558,7 +558,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         l1 = block_sum(l1, scratch);
         l2 = block_sum(l2, scratch);
         if (threadIdx.x == 0) {
-            double *p = a.part + (size_t)blockIdx.x * RX_N;
+            double *p = a.part + (size_t)blockIdx.y * a.bt.part + (size_t)blockIdx.x * RX_N;
             p[RX_W2MAX] = w2max;
             p[RX_GG] = gg;
             p[RX_USGG] = usgg;
@@ -597,11 +597,12 @@ hipError_t configure_linepass() {
 template <int N>
 static void launch_line_t(const LineArgs &a, hipStream_t stream) {
     const int blocks = a.nother * a.nzt;
+    const int nb = a.bt.nb > 1 ? a.bt.nb : 1;
     const size_t lds = line_lds_bytes(a.n, a.ZC);
     if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF)
-        hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks), dim3(256), lds, stream, a);
+        hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks, nb), dim3(256), lds, stream, a);
     else
-        hipLaunchKernelGGL(k_line_reg<N>, dim3(blocks), dim3(kLineThreads), lds, stream, a);
+        hipLaunchKernelGGL(k_line_reg<N>, dim3(blocks, nb), dim3(kLineThreads), lds, stream, a);
 }
 
 hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
@@ -622,22 +623,24 @@ hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_fin_y(FinArgs a) {
     __shared__ double red[256];
+    const double *party = a.party + (size_t)blockIdx.y * a.bt.party;
+    double *scal = a.scal + (size_t)blockIdx.y * SC_NSCAL;
     double s[RY_N];
     for (int q = 0; q < RY_N; ++q) s[q] = 0.0;
     for (int b = threadIdx.x; b < a.nby; b += blockDim.x)
-        for (int q = 0; q < RY_N; ++q) s[q] += a.party[(size_t)b * RY_N + q];
+        for (int q = 0; q < RY_N; ++q) s[q] += party[(size_t)b * RY_N + q];
     double tot[RY_N];
     for (int q = 0; q < RY_N; ++q) tot[q] = block_sum(s[q], red);
     if (threadIdx.x == 0) {
         // get_kinetic_energy / get_enstrophy: (1/2 interior + 1/4 lids) / ncell
         // (field_diagnostics.f90:85-206); vortrms = sqrt(get_mean(|w|^2)) (advance.f90:177)
-        a.scal[SC_KE] = 0.5 * tot[RY_UU] * a.ncelli;
-        a.scal[SC_EN] = 0.5 * tot[RY_WW] * a.ncelli;
-        a.scal[SC_HEL] = tot[RY_UW] * a.ncelli;
-        a.scal[SC_VORTRMS] = sqrt(tot[RY_WW] * a.ncelli);
-        a.scal[SC_VORMX] = tot[RY_MX] * a.ncelli;
-        a.scal[SC_VORMY] = tot[RY_MY] * a.ncelli;
-        a.scal[SC_VORMZ] = tot[RY_MZ] * a.ncelli;
+        scal[SC_KE] = 0.5 * tot[RY_UU] * a.ncelli;
+        scal[SC_EN] = 0.5 * tot[RY_WW] * a.ncelli;
+        scal[SC_HEL] = tot[RY_UW] * a.ncelli;
+        scal[SC_VORTRMS] = sqrt(tot[RY_WW] * a.ncelli);
+        scal[SC_VORMX] = tot[RY_MX] * a.ncelli;
+        scal[SC_VORMY] = tot[RY_MY] * a.ncelli;
+        scal[SC_VORMZ] = tot[RY_MZ] * a.ncelli;
     }
 }
 
@@ -645,8 +648,11 @@ __global__ void __launch_bounds__(256) k_fin_x(FinArgs a) {
     __shared__ double red[256];
     double w2 = 0.0, gg = 2.220446049250313e-16, us = 0.0, ls = 0.0;
     double um = -INFINITY, vm = -INFINITY, wm = -INFINITY, l1 = 0.0, l2 = 0.0;
+    const double *partx = a.partx + (size_t)blockIdx.y * a.bt.partx;
+    double *roll_hist = a.roll_hist + (size_t)blockIdx.y * a.bt.roll;
+    double *diag = a.diag ? a.diag + (size_t)blockIdx.y * a.bt.diag : nullptr;
     for (int b = threadIdx.x; b < a.nbx; b += blockDim.x) {
-        const double *p = a.partx + (size_t)b * RX_N;
+        const double *p = partx + (size_t)b * RX_N;
         w2 = fmax(w2, p[RX_W2MAX]);
         gg = fmax(gg, p[RX_GG]);
         us = fmax(us, p[RX_USGG]);
@@ -667,7 +673,7 @@ __global__ void __launch_bounds__(256) k_fin_x(FinArgs a) {
     l1 = block_sum(l1, red);
     l2 = block_sum(l2, red);
     if (threadIdx.x != 0) return;
-    double *sc = a.scal;
+    double *sc = a.scal + (size_t)blockIdx.y * SC_NSCAL;
     const double small = 1.0e-12, cflmax = 0.8;
     const double t = sc[SC_T];
     const double vortmax = sqrt(w2);
@@ -682,15 +688,15 @@ __global__ void __launch_bounds__(256) k_fin_x(FinArgs a) {
     int inew = (int)sc[SC_RM_INEW], iold = (int)sc[SC_RM_IOLD], filled = (int)sc[SC_RM_FILLED];
     const int len = a.roll_len;
     if (filled) {
-        const double vold = a.roll_hist[iold - 1];
+        const double vold = roll_hist[iold - 1];
         iold = (iold % len) + 1;
         sma = sma + (gg - vold) / (double)len;
-        a.roll_hist[inew - 1] = gg;
+        roll_hist[inew - 1] = gg;
         inew = (inew % len) + 1;
     } else {
-        a.roll_hist[inew - 1] = gg;
+        roll_hist[inew - 1] = gg;
         double s = 0.0;
-        for (int i = 0; i < inew; ++i) s += a.roll_hist[i];
+        for (int i = 0; i < inew; ++i) s += roll_hist[i];
         sma = s / (double)inew;
         filled = (len == inew);
         inew = (inew % len) + 1;
@@ -715,10 +721,10 @@ __global__ void __launch_bounds__(256) k_fin_x(FinArgs a) {
         sc[SC_DFAC] = (a.nnu == 1) ? dt : pref * dt;    // cn2.f90:52-58
     else
         sc[SC_DFAC] = 0.5 * pref * dt;                   // impl_rk4.f90:43
-    if (a.diag) {
+    if (diag) {
         const int row = (int)sc[SC_STEP];
         if (row < a.max_rows) {
-            double *d = a.diag + (size_t)row * 19;
+            double *d = diag + (size_t)row * 19;
             d[0] = t; d[1] = dt; d[2] = sc[SC_KE]; d[3] = sc[SC_EN]; d[4] = sc[SC_HEL];
             d[5] = vortmax; d[6] = sc[SC_VORTRMS]; d[7] = vorch;
             d[8] = sc[SC_VORMX]; d[9] = sc[SC_VORMY]; d[10] = sc[SC_VORMZ];
@@ -729,11 +735,11 @@ __global__ void __launch_bounds__(256) k_fin_x(FinArgs a) {
 }
 
 hipError_t launch_fin_y(const FinArgs &a, hipStream_t stream) {
-    hipLaunchKernelGGL(k_fin_y, dim3(1), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_fin_y, dim3(1, a.bt.nb > 1 ? a.bt.nb : 1), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream) {
-    hipLaunchKernelGGL(k_fin_x, dim3(1), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_fin_x, dim3(1, a.bt.nb > 1 ? a.bt.nb : 1), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

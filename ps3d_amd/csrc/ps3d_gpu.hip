@@ -65,6 +65,11 @@ struct ps3d_handle {
     LineCfg lx{}, ly{};
     double visc = 0.0;
     bool have_state = false, have_phys = false, have_adapt = false;
+    // batched instances of this grid (ensemble group): instance b's data sit at
+    // b * stride from instance 0; launches of the stepping kernels cover all of them
+    int nbatch = 1;
+    long b_vol = 0, b_party = 0, b_partx = 0, b_roll = 0, b_diag = 0;
+    std::vector<char> inst_set;    // instances given an initial vorticity
     bool have_grad = false;   // grad holds the physical y-derivatives of u, v, w (vor2vel, before source)
     bool have_derived = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -205,8 +210,10 @@ static void free_handle(ps3d_handle *h) {
 }
 
 // Builds tables, plans and launch shapes. alloc_fields = false for the 1-D helpers.
-static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, bool strict) {
+static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, bool strict, int nbatch = 1) {
     h->p = *p;
+    h->nbatch = nbatch;
+    h->inst_set.assign(nbatch, 0);
     const int nx = p->nx, ny = p->ny, nz = p->nz;
     if (strict) {
         if (nx < 2 || ny < 2 || nz < 2 || (nx & 1) || (ny & 1) || (nz & 1))
@@ -309,14 +316,19 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     HIPCHK(ps3d::configure_zwave());
     HIPCHK(ps3d::configure_linepass());
 
-    HIPCHK(hipMalloc((void **)&h->scal, sizeof(double) * SC_NSCAL));
-    HIPCHK(hipMemset(h->scal, 0, sizeof(double) * SC_NSCAL));
-    HIPCHK(hipMalloc((void **)&h->party, sizeof(double) * (size_t)h->ly.nother * h->ly.nzt * RY_N));
-    HIPCHK(hipMalloc((void **)&h->partx, sizeof(double) * (size_t)h->lx.nother * h->lx.nzt * RX_N));
-    HIPCHK(hipMalloc((void **)&h->roll, sizeof(double) * std::max(1, p->roll_mean_win_size)));
+    const size_t nb = (size_t)nbatch;
+    h->b_party = (long)h->ly.nother * h->ly.nzt * RY_N;
+    h->b_partx = (long)h->lx.nother * h->lx.nzt * RX_N;
+    h->b_roll = std::max(1, p->roll_mean_win_size);
+    HIPCHK(hipMalloc((void **)&h->scal, sizeof(double) * SC_NSCAL * nb));
+    HIPCHK(hipMemset(h->scal, 0, sizeof(double) * SC_NSCAL * nb));
+    HIPCHK(hipMalloc((void **)&h->party, sizeof(double) * h->b_party * nb));
+    HIPCHK(hipMalloc((void **)&h->partx, sizeof(double) * h->b_partx * nb));
+    HIPCHK(hipMalloc((void **)&h->roll, sizeof(double) * h->b_roll * nb));
     if (alloc_fields) {
         const size_t nv = 24;
-        const size_t bytes = sizeof(double) * h->nvol * nv;
+        h->b_vol = (long)(h->nvol * nv);
+        const size_t bytes = sizeof(double) * h->nvol * nv * nb;
         hipError_t e = hipMalloc((void **)&h->pool, bytes);
         if (e != hipSuccess) return fail(PS3D_E_NOMEM, "device allocation of the field pool failed");
         HIPCHK(hipMemset(h->pool, 0, bytes));
@@ -331,9 +343,22 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
 // ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
+static Batch batch(const ps3d_handle *h) {
+    Batch b;
+    b.nb = h->nbatch;
+    b.vol = h->b_vol;
+    b.part = 0;
+    b.party = h->b_party;
+    b.partx = h->b_partx;
+    b.diag = h->b_diag;
+    b.roll = h->b_roll;
+    return b;
+}
+
 static LineArgs line_args(const ps3d_handle *h, int axis, int mode) {
     LineArgs a;
     std::memset(&a, 0, sizeof a);
+    a.bt = batch(h);
     const LineCfg &c = axis == 0 ? h->lx : h->ly;
     a.g = h->g;
     a.axis = axis;
@@ -392,6 +417,7 @@ static FinArgs fin_args(ps3d_handle *h) {
     f.nnu = h->p.nnu;
     f.roll_len = h->p.roll_mean_win_size;
     f.roll_hist = h->roll;
+    f.bt = batch(h);
     return f;
 }
 
@@ -407,6 +433,7 @@ static ZInvArgs zinv_args(ps3d_handle *h) {
     a.twz = h->twz;
     a.zscale = 1.0 / std::sqrt(2.0 * h->g.nz);
     a.rkz_sc = 3.141592653589793238462643383279502884 / h->p.extent[2];
+    a.bt = batch(h);
     for (int c = 0; c < 3; ++c) a.svor[c] = h->svor[c];
     for (int c = 0; c < 3; ++c) {
         a.out[c] = h->svel[c];
@@ -436,6 +463,7 @@ static ZTendArgs ztend_args(ps3d_handle *h, int mode, int stage, double *const *
     a.rk.scal = h->scal;
     a.mode = mode;
     a.stage = stage;
+    a.bt = batch(h);
     return a;
 }
 
@@ -469,6 +497,7 @@ static int run_mean(ps3d_handle *h, int mode, int advance_time) {
     a.scal = h->scal;
     a.mode = mode;
     a.advance_time = advance_time;
+    a.bt = batch(h);
     prof_begin(h, PS3D_K_MEAN);
     HIPCHK(launch_mean(a, h->g.nz, h->stream));
     prof_end(h);
@@ -499,6 +528,7 @@ static int inverse_adapt_stage(ps3d_handle *h, bool project) {
         y.out[9 + c] = h->grad[c];
     }
     y.part = h->party;
+    y.bt.part = h->b_party;
     prof_begin(h, PS3D_K_YINV_ADAPT);
     int st = run_line(h, y);
     prof_end(h);
@@ -519,6 +549,7 @@ static int x_adapt(ps3d_handle *h, int phys_in) {
         x.in[9 + c] = h->mid[c];
     }
     x.part = h->partx;
+    x.bt.part = h->b_partx;
     prof_begin(h, PS3D_K_XADAPT);
     int st = run_line(h, x);
     prof_end(h);
@@ -617,9 +648,50 @@ static int ensure_diag(ps3d_handle *h, int rows) {
     if (rows <= h->diag_cap) return 0;
     hipFree(h->diag);
     h->diag = nullptr;
-    HIPCHK(hipMalloc((void **)&h->diag, sizeof(double) * PS3D_NDIAG * (size_t)rows));
+    HIPCHK(hipMalloc((void **)&h->diag, sizeof(double) * PS3D_NDIAG * (size_t)rows * h->nbatch));
     h->diag_cap = rows;
+    h->b_diag = (long)PS3D_NDIAG * rows;
     return 0;
+}
+
+// A single-instance view of instance b of a batched handle: the same tables,
+// stream and launch shapes, data pointers moved to the instance.  Used for the
+// per-instance set-up and copies; never freed.
+static ps3d_handle make_view(const ps3d_handle *h, int b) {
+    ps3d_handle v;
+    v.p = h->p;
+    v.device = h->device;
+    v.stream = h->stream;
+    v.g = h->g;
+    v.nvol = h->nvol;
+    const long o = (long)b * h->b_vol;
+    for (int c = 0; c < 3; ++c) {
+        v.svor[c] = h->svor[c] + o;
+        v.qdi[c] = h->qdi[c] + o;
+        v.qdf[c] = h->qdf[c] + o;
+        v.svel[c] = h->svel[c] + o;
+        v.vel[c] = h->vel[c] + o;
+        v.vor[c] = h->vor[c] + o;
+        v.grad[c] = h->grad[c] + o;
+        v.mid[c] = h->mid[c] + o;
+    }
+    v.scal = h->scal + (size_t)b * SC_NSCAL;
+    v.party = h->party + (size_t)b * h->b_party;
+    v.partx = h->partx + (size_t)b * h->b_partx;
+    v.roll = h->roll + (size_t)b * h->b_roll;
+    v.diag = h->diag ? h->diag + (size_t)b * h->b_diag : nullptr;
+    v.diag_cap = h->diag_cap;
+    v.rkx = h->rkx; v.rky = h->rky; v.filx = h->filx; v.fily = h->fily; v.filz = h->filz;
+    v.twx = h->twx; v.twy = h->twy; v.twz = h->twz;
+    v.px = h->px; v.py = h->py; v.pz = h->pz;
+    v.zl = h->zl;
+    v.TQ = h->TQ; v.QPB = h->QPB; v.nquads = h->nquads;
+    v.lx = h->lx; v.ly = h->ly;
+    v.h_rkx = h->h_rkx;
+    v.h_rky = h->h_rky;
+    v.nbatch = 1;
+    v.have_state = true;
+    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -674,6 +746,7 @@ int ps3d_gpu_sync(ps3d_handle *h) {
 
 // setup_fields (utils.f90:136-184)
 int ps3d_gpu_set_vorticity(ps3d_handle *h, const double *vor) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !vor) return fail(PS3D_E_ARG, "null argument");
     const size_t hv = (size_t)h->g.nx * h->g.ny * (h->g.nz + 1);
     int st;
@@ -707,6 +780,7 @@ int ps3d_gpu_set_vorticity(ps3d_handle *h, const double *vor) {
 
 // vor2vel: device state -> physical vel, vor (and y-derivatives / midpoints for adapt)
 int ps3d_gpu_vor2vel(ps3d_handle *h) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h) return fail(PS3D_E_ARG, "null handle");
     if (!h->have_state) return fail(PS3D_E_STATE, "vor2vel before set_vorticity");
     int st;
@@ -726,6 +800,7 @@ int ps3d_gpu_vor2vel(ps3d_handle *h) {
 }
 
 int ps3d_gpu_adapt(ps3d_handle *h, double t, double *dt, double *diag) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h) return fail(PS3D_E_ARG, "null handle");
     if (!h->have_phys) return fail(PS3D_E_STATE, "adapt needs a preceding vor2vel");
     int st;
@@ -744,6 +819,7 @@ int ps3d_gpu_adapt(ps3d_handle *h, double t, double *dt, double *diag) {
 }
 
 int ps3d_gpu_source(ps3d_handle *h) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h) return fail(PS3D_E_ARG, "null handle");
     if (!h->have_phys) return fail(PS3D_E_STATE, "source needs a preceding vor2vel");
     int st;
@@ -756,6 +832,7 @@ int ps3d_gpu_source(ps3d_handle *h) {
 }
 
 int ps3d_gpu_step(ps3d_handle *h, double *t, double dt) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !t) return fail(PS3D_E_ARG, "null argument");
     if (!h->have_adapt) return fail(PS3D_E_STATE, "step needs vor2vel, adapt and source first");
     int st;
@@ -785,6 +862,7 @@ int ps3d_gpu_step(ps3d_handle *h, double *t, double dt) {
 }
 
 int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_series) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !t || nsteps < 0) return fail(PS3D_E_ARG, "invalid argument");
     if (!h->have_state) return fail(PS3D_E_STATE, "advance before set_vorticity");
     int st;
@@ -813,6 +891,7 @@ int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_serie
 }
 
 int ps3d_gpu_rk_substep(ps3d_handle *h, int stage, double dt, const double *vdiss) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !vdiss || stage < 1 || stage > 4) return fail(PS3D_E_ARG, "invalid argument");
     const size_t n2 = (size_t)h->g.nx * h->g.ny;
     double *dv = nullptr;
@@ -829,6 +908,7 @@ int ps3d_gpu_rk_substep(ps3d_handle *h, int stage, double dt, const double *vdis
 }
 
 int ps3d_gpu_profile_kernels(ps3d_handle *h, int nsteps, double *ms, int *launches, double *bytes) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || nsteps < 1) return fail(PS3D_E_ARG, "invalid argument");
     if (!h->have_state) return fail(PS3D_E_STATE, "profile before set_vorticity");
     int st;
@@ -883,6 +963,7 @@ int ps3d_gpu_last_timing(ps3d_handle *h, double *ms, int *nsteps) {
 // last vor2vel: svel (semi-spectral), vor and the y-derivatives du/dy, dv/dy, dw/dy
 // (physical).  qdi / qdf serve as scratch: they only carry data inside a step.
 int ps3d_gpu_derived(ps3d_handle *h) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h) return fail(PS3D_E_ARG, "null handle");
     if (!h->have_phys || !h->have_grad)
         return fail(PS3D_E_STATE, "derived needs a preceding vor2vel (and no source since)");
@@ -927,6 +1008,7 @@ int ps3d_gpu_derived(ps3d_handle *h) {
 }
 
 int ps3d_gpu_get(ps3d_handle *h, int which, double *host) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !host) return fail(PS3D_E_ARG, "null argument");
     const size_t hv = (size_t)h->g.nx * h->g.ny * (h->g.nz + 1);
     double *const *src = nullptr;
@@ -952,6 +1034,7 @@ int ps3d_gpu_get(ps3d_handle *h, int which, double *host) {
 }
 
 int ps3d_gpu_set(ps3d_handle *h, int which, const double *host) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !host) return fail(PS3D_E_ARG, "null argument");
     const size_t hv = (size_t)h->g.nx * h->g.ny * (h->g.nz + 1);
     double *const *dst = nullptr;
@@ -968,6 +1051,7 @@ int ps3d_gpu_set(ps3d_handle *h, int which, const double *host) {
 }
 
 int ps3d_gpu_op(ps3d_handle *h, int op, const double *in, double *out) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !in || !out) return fail(PS3D_E_ARG, "null argument");
     double *A = h->grad[0], *B = h->grad[1];
     h->have_grad = false;
@@ -1085,6 +1169,304 @@ int ps3d_gpu_fft1d(int kind, int m, int n, double *x) {
     free_handle(h);
     g_err = msg;
     return st;
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched instances of one grid, and ensembles of several grids
+// ---------------------------------------------------------------------------
+int ps3d_gpu_create_batch(const ps3d_params *p, int nbatch, ps3d_handle **out) {
+    if (!p || !out || nbatch < 1) return fail(PS3D_E_ARG, "invalid argument");
+    *out = nullptr;
+    ps3d_handle *h = new ps3d_handle();
+    int st = init_handle(h, p, true, true, nbatch);
+    if (st) {
+        std::string msg = g_err;
+        free_handle(h);
+        g_err = msg;
+        return st;
+    }
+    *out = h;
+    return 0;
+}
+
+int ps3d_gpu_batch_size(ps3d_handle *h, int *nbatch) {
+    if (!h || !nbatch) return fail(PS3D_E_ARG, "null argument");
+    *nbatch = h->nbatch;
+    return 0;
+}
+
+int ps3d_gpu_set_vorticity_batch(ps3d_handle *h, int b, const double *vor) {
+    if (!h || !vor || b < 0 || b >= h->nbatch) return fail(PS3D_E_ARG, "invalid argument");
+    ps3d_handle v = make_view(h, b);
+    v.have_state = false;
+    int st = ps3d_gpu_set_vorticity(&v, vor);
+    if (st) return st;
+    h->inst_set[b] = 1;
+    h->have_state = std::all_of(h->inst_set.begin(), h->inst_set.end(), [](char c) { return c != 0; });
+    h->have_phys = h->have_adapt = h->have_grad = false;
+    return 0;
+}
+
+int ps3d_gpu_get_batch(ps3d_handle *h, int b, int which, double *host) {
+    if (!h || !host || b < 0 || b >= h->nbatch) return fail(PS3D_E_ARG, "invalid argument");
+    if (which != PS3D_SVOR) return fail(PS3D_E_UNSUPPORTED, "batched handles hold svor between steps");
+    ps3d_handle v = make_view(h, b);
+    return ps3d_gpu_get(&v, which, host);
+}
+
+// Enqueues nsteps fused steps of every instance (no host synchronisation).
+static int enqueue_steps(ps3d_handle *h, int nsteps) {
+    int st;
+    for (int s = 0; s < nsteps; ++s)
+        if ((st = one_step(h))) return st;
+    return 0;
+}
+
+static int put_times(ps3d_handle *h, const double *t) {
+    std::vector<double> sc((size_t)SC_NSCAL * h->nbatch);
+    HIPCHK(hipMemcpyAsync(sc.data(), h->scal, sizeof(double) * sc.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int b = 0; b < h->nbatch; ++b) {
+        sc[(size_t)b * SC_NSCAL + SC_T] = t[b];
+        sc[(size_t)b * SC_NSCAL + SC_STEP] = 0.0;
+    }
+    HIPCHK(hipMemcpyAsync(h->scal, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static int get_times(ps3d_handle *h, double *t) {
+    std::vector<double> sc((size_t)SC_NSCAL * h->nbatch);
+    HIPCHK(hipMemcpyAsync(sc.data(), h->scal, sizeof(double) * sc.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int b = 0; b < h->nbatch; ++b) t[b] = sc[(size_t)b * SC_NSCAL + SC_T];
+    return 0;
+}
+
+int ps3d_gpu_advance_n_batch(ps3d_handle *h, double *t, int nsteps, double *diag_series) {
+    if (!h || !t || nsteps < 0) return fail(PS3D_E_ARG, "invalid argument");
+    if (!h->have_state) return fail(PS3D_E_STATE, "advance before every instance has an initial vorticity");
+    int st;
+    if (diag_series && (st = ensure_diag(h, std::max(1, nsteps)))) return st;
+    if ((st = put_times(h, t))) return st;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    if ((st = enqueue_steps(h, nsteps))) return st;
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    h->last_steps = nsteps;
+    if ((st = get_times(h, t))) return st;
+    if (diag_series && nsteps > 0)
+        HIPCHK(hipMemcpy2D(diag_series, sizeof(double) * PS3D_NDIAG * nsteps, h->diag, sizeof(double) * h->b_diag,
+                           sizeof(double) * PS3D_NDIAG * nsteps, h->nbatch, hipMemcpyDeviceToHost));
+    h->have_phys = h->have_adapt = h->have_grad = false;
+    return 0;
+}
+
+struct ps3d_ensemble {
+    std::vector<ps3d_handle *> g;
+    hipStream_t s0 = nullptr;
+    hipEvent_t fork = nullptr, ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> join;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double last_ms = 0.0;
+};
+
+static void drop_graph(ps3d_ensemble *e) {
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+int ps3d_gpu_ensemble_destroy(ps3d_ensemble *e) {
+    if (!e) return 0;
+    if (e->s0) hipStreamSynchronize(e->s0);
+    drop_graph(e);
+    for (ps3d_handle *h : e->g) free_handle(h);
+    for (hipEvent_t ev : e->join) hipEventDestroy(ev);
+    if (e->fork) hipEventDestroy(e->fork);
+    if (e->ev0) hipEventDestroy(e->ev0);
+    if (e->ev1) hipEventDestroy(e->ev1);
+    if (e->s0) hipStreamDestroy(e->s0);
+    delete e;
+    return 0;
+}
+
+int ps3d_gpu_ensemble_create(int ngroups, const ps3d_params *p, const int *nbatch, ps3d_ensemble **out) {
+    if (ngroups < 1 || !p || !nbatch || !out) return fail(PS3D_E_ARG, "invalid argument");
+    *out = nullptr;
+    ps3d_ensemble *e = new ps3d_ensemble();
+    auto bail = [&](int st) {
+        std::string msg = g_err;
+        ps3d_gpu_ensemble_destroy(e);
+        g_err = msg;
+        return st;
+    };
+    for (int i = 0; i < ngroups; ++i) {
+        ps3d_handle *h = nullptr;
+        int st = ps3d_gpu_create_batch(&p[i], nbatch[i], &h);
+        if (st) return bail(st);
+        e->g.push_back(h);
+        hipEvent_t ev;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+            return bail(fail(PS3D_E_HIP, "hipEventCreate failed"));
+        e->join.push_back(ev);
+    }
+    if (hipStreamCreateWithFlags(&e->s0, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&e->ev0) != hipSuccess ||
+        hipEventCreate(&e->ev1) != hipSuccess)
+        return bail(fail(PS3D_E_HIP, "stream / event creation failed"));
+    *out = e;
+    return 0;
+}
+
+int ps3d_gpu_ensemble_group(ps3d_ensemble *e, int g, ps3d_handle **h) {
+    if (!e || !h || g < 0 || g >= (int)e->g.size()) return fail(PS3D_E_ARG, "invalid argument");
+    *h = e->g[g];
+    return 0;
+}
+
+// One step of every group: the groups' streams fork from s0 and join back, so the
+// groups run concurrently.
+static int enqueue_ensemble_step(ps3d_ensemble *e) {
+    HIPCHK(hipEventRecord(e->fork, e->s0));
+    for (size_t i = 0; i < e->g.size(); ++i) {
+        ps3d_handle *h = e->g[i];
+        HIPCHK(hipStreamWaitEvent(h->stream, e->fork, 0));
+        int st = one_step(h);
+        if (st) return st;
+        HIPCHK(hipEventRecord(e->join[i], h->stream));
+        HIPCHK(hipStreamWaitEvent(e->s0, e->join[i], 0));
+    }
+    return 0;
+}
+
+static int build_graph(ps3d_ensemble *e) {
+    drop_graph(e);
+    HIPCHK(hipStreamBeginCapture(e->s0, hipStreamCaptureModeRelaxed));
+    int st = enqueue_ensemble_step(e);
+    hipGraph_t gr = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->s0, &gr);
+    if (st) {
+        if (gr) hipGraphDestroy(gr);
+        return st;
+    }
+    if (ce != hipSuccess) return fail(PS3D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+    e->graph = gr;
+    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+int ps3d_gpu_ensemble_advance(ps3d_ensemble *e, int nsteps, int use_graph, double *ms) {
+    if (!e || nsteps < 0) return fail(PS3D_E_ARG, "invalid argument");
+    for (ps3d_handle *h : e->g)
+        if (!h->have_state) return fail(PS3D_E_STATE, "advance before every instance has an initial vorticity");
+    int st;
+    if (use_graph && !e->exec && (st = build_graph(e))) return st;
+    HIPCHK(hipEventRecord(e->ev0, e->s0));
+    for (int s = 0; s < nsteps; ++s) {
+        if (use_graph)
+            HIPCHK(hipGraphLaunch(e->exec, e->s0));
+        else if ((st = enqueue_ensemble_step(e)))
+            return st;
+    }
+    HIPCHK(hipEventRecord(e->ev1, e->s0));
+    HIPCHK(hipStreamSynchronize(e->s0));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, e->ev0, e->ev1));
+    e->last_ms = t;
+    if (ms) *ms = t;
+    for (ps3d_handle *h : e->g) h->have_phys = h->have_adapt = h->have_grad = false;
+    return 0;
+}
+
+// Launch-shape candidates of one grid: the z kernels (one wave per column where
+// available, else quad-per-thread with QPB quads per block).  Code = wave * 100 + QPB.
+static std::vector<int> z_candidates(const ps3d_handle *h) {
+    std::vector<int> c;
+    if (ps3d::zwave_supported(h->g.nz)) c.push_back(100);
+    for (int q = 1; q <= 8; q *= 2) {
+        if (h->TQ * q > 512) break;
+        if (zpass_lds_bytes(q, h->g.nz) > 160 * 1024) break;
+        c.push_back(q);
+    }
+    return c;
+}
+
+static void apply_z_choice(ps3d_handle *h, int code) {
+    h->zl.wave = code >= 100 ? 1 : 0;
+    if (!h->zl.wave) {
+        h->QPB = code;
+        h->zl.threads = h->TQ * h->QPB;
+        h->zl.blocks = (h->nquads + h->QPB - 1) / h->QPB;
+        h->zl.lds = zpass_lds_bytes(h->QPB, h->g.nz);
+    }
+}
+
+static int current_z_choice(const ps3d_handle *h) { return h->zl.wave ? 100 : h->QPB; }
+
+// Times `steps` fused steps of each group under every candidate shape and keeps the
+// fastest; the state (svor, device scalars, rolling-mean history) is restored.
+int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice) {
+    if (!e || steps < 1) return fail(PS3D_E_ARG, "invalid argument");
+    int st;
+    for (size_t gi = 0; gi < e->g.size(); ++gi) {
+        ps3d_handle *h = e->g[gi];
+        if (!h->have_state) return fail(PS3D_E_STATE, "tune needs the initial vorticity of every instance");
+        const size_t nb = h->nbatch, sv = 3 * h->nvol;
+        double *bk = nullptr;
+        HIPCHK(hipMalloc((void **)&bk, sizeof(double) * (sv * nb + SC_NSCAL * nb + h->b_roll * nb)));
+        double *bsc = bk + sv * nb, *brl = bsc + SC_NSCAL * nb;
+        for (size_t b = 0; b < nb; ++b)
+            HIPCHK(hipMemcpyAsync(bk + b * sv, h->svor[0] + b * h->b_vol, sizeof(double) * sv, hipMemcpyDeviceToDevice,
+                                  h->stream));
+        HIPCHK(hipMemcpyAsync(bsc, h->scal, sizeof(double) * SC_NSCAL * nb, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(brl, h->roll, sizeof(double) * h->b_roll * nb, hipMemcpyDeviceToDevice, h->stream));
+        const std::vector<int> cand = z_candidates(h);
+        int best = current_z_choice(h);
+        float best_ms = 1e30f;
+        for (int code : cand) {
+            apply_z_choice(h, code);
+            if ((st = enqueue_steps(h, 1))) break;       // warm-up
+            HIPCHK(hipEventRecord(h->ev0, h->stream));
+            if ((st = enqueue_steps(h, steps))) break;
+            HIPCHK(hipEventRecord(h->ev1, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = code;
+            }
+        }
+        apply_z_choice(h, best);
+        for (size_t b = 0; b < nb; ++b)
+            HIPCHK(hipMemcpyAsync(h->svor[0] + b * h->b_vol, bk + b * sv, sizeof(double) * sv, hipMemcpyDeviceToDevice,
+                                  h->stream));
+        HIPCHK(hipMemcpyAsync(h->scal, bsc, sizeof(double) * SC_NSCAL * nb, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemcpyAsync(h->roll, brl, sizeof(double) * h->b_roll * nb, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        hipFree(bk);
+        if (st) return st;
+        if (choice) choice[gi] = best;
+    }
+    drop_graph(e);
+    return 0;
+}
+
+int ps3d_gpu_ensemble_set_choice(ps3d_ensemble *e, int g, int code) {
+    if (!e || g < 0 || g >= (int)e->g.size()) return fail(PS3D_E_ARG, "invalid argument");
+    ps3d_handle *h = e->g[g];
+    const std::vector<int> cand = z_candidates(h);
+    if (std::find(cand.begin(), cand.end(), code) == cand.end()) return fail(PS3D_E_ARG, "launch shape not available");
+    apply_z_choice(h, code);
+    drop_graph(e);
+    return 0;
 }
 
 }  // extern "C"

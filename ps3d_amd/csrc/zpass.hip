@@ -39,7 +39,7 @@ struct ZT {
 };
 
 template <int ZPT>
-__device__ inline ZT<ZPT> make_zt(const Grid &g, const ZTabs &tb, int TQ, int QPB, int nquads) {
+__device__ inline ZT<ZPT> make_zt(const Grid &g, const ZTabs &tb, int TQ, int QPB, int nquads, long ioff = 0) {
     ZT<ZPT> T;
     T.TQ = TQ;
     T.QPB = QPB;
@@ -68,7 +68,7 @@ __device__ inline ZT<ZPT> make_zt(const Grid &g, const ZTabs &tb, int TQ, int QP
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         T.colidx[m] = (size_t)mx[m] * g.ny + my[m];
-        T.col[m] = T.colidx[m] * (size_t)g.zs;
+        T.col[m] = T.colidx[m] * (size_t)g.zs + ioff;   // ioff: batched instance
         T.store[m] = dist[m] && T.qvalid;
         // diffx: ds(k) = si * rk(k) * fs(n-k), si = +1 for k > n/2, -1 otherwise;
         // ds(0) = 0 and ds(n/2) = 0 for even n (sta3dfft.f90:318-335)
@@ -517,7 +517,7 @@ __device__ inline void zstore(const ZT<ZPT> &T, double *__restrict__ dst, const 
 template <int ZPT, int NZ>
 __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     const Grid g = a.g;
-    ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads);
+    ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads, (long)blockIdx.y * a.bt.vol);
     ZLds L = zlds(a.QPB, T.N, g.nz);
     const FftPlan &P = a.pz;
     const double sc = a.zscale;
@@ -665,7 +665,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
 template <int ZPT>
 __device__ inline void rk_update(const ZT<ZPT> &T, const RkArgs &rk, int mode, int stage, double (&S)[3][ZPT][4],
                                  const ZTabs &tb) {
-    const double *sc = rk.scal;
+    const double *sc = rk.scal + (size_t)blockIdx.y * SC_NSCAL;
     const double dt = sc[SC_DT];
     // filt(0,ky,kx) with filt(:,0,0) = 1 (inversion_utils.f90:263-277, 377-455)
     double f0;
@@ -754,7 +754,7 @@ __device__ inline void rk_update(const ZT<ZPT> &T, const RkArgs &rk, int mode, i
 template <int ZPT, int NZ>
 __global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
     const Grid g = a.g;
-    ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads);
+    ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads, (long)blockIdx.y * a.bt.vol);
     ZLds L = zlds(a.QPB, T.N, g.nz);
     const FftPlan &P = a.pz;
     const double sc = a.zscale;
@@ -806,7 +806,7 @@ __global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
 template <int ZPT>
 __global__ void __launch_bounds__(512) k_zrk(ZTendArgs a) {
     const Grid g = a.g;
-    ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads);
+    ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads, (long)blockIdx.y * a.bt.vol);
     double S[3][ZPT][4];
     for (int c = 0; c < 3; ++c) zload(T, a.svorts[c], S[c]);
     rk_update(T, a.rk, a.mode, a.stage, S, a.tb);
@@ -909,7 +909,9 @@ __global__ void __launch_bounds__(256) k_mean(MeanArgs a) {
     const int n = a.g.nz, N = 2 * n, tid = threadIdx.x;
     double2 *A = msmem, *B = msmem + N;
     double *red = reinterpret_cast<double *>(B + N);
-    double *xi = a.svor[0], *eta = a.svor[1];   // column (0,0) starts at offset 0
+    // column (0,0) starts at offset 0 of each instance
+    double *xi = a.svor[0] + (size_t)blockIdx.y * a.bt.vol, *eta = a.svor[1] + (size_t)blockIdx.y * a.bt.vol;
+    double *scal = a.scal + (size_t)blockIdx.y * SC_NSCAL;
     for (int z = tid; z <= n; z += blockDim.x) {
         if (z == 0 || z == n) {
             A[z] = make_double2(0.0, 0.0);
@@ -943,10 +945,10 @@ __global__ void __launch_bounds__(256) k_mean(MeanArgs a) {
         const double savg0 = 0.5 * (xi[0] + xi[n]) + fnzi * red[0];
         const double savg1 = 0.5 * (eta[0] + eta[n]) + fnzi * red[blockDim.x];
         if (a.mode == MEAN_CALC) {
-            a.scal[SC_INI_XI] = savg0;
-            a.scal[SC_INI_ETA] = savg1;
+            scal[SC_INI_XI] = savg0;
+            scal[SC_INI_ETA] = savg1;
         } else {
-            const double d0 = a.scal[SC_INI_XI] - savg0, d1 = a.scal[SC_INI_ETA] - savg1;
+            const double d0 = scal[SC_INI_XI] - savg0, d1 = scal[SC_INI_ETA] - savg1;
             xi[0] = xi[0] + d0;
             xi[n] = xi[n] + d0;
             eta[0] = eta[0] + d1;
@@ -954,10 +956,10 @@ __global__ void __launch_bounds__(256) k_mean(MeanArgs a) {
         }
         // impl_rk4 advances t by dt/2 twice (impl_rk4.f90:123,149), cn2 by dt (cn2.f90:179)
         if (a.advance_time == 1) {
-            const double dt2 = 0.5 * a.scal[SC_DT];
-            a.scal[SC_T] = (a.scal[SC_T] + dt2) + dt2;
+            const double dt2 = 0.5 * scal[SC_DT];
+            scal[SC_T] = (scal[SC_T] + dt2) + dt2;
         } else if (a.advance_time == 2) {
-            a.scal[SC_T] = a.scal[SC_T] + a.scal[SC_DT];
+            scal[SC_T] = scal[SC_T] + scal[SC_DT];
         }
     }
 }
@@ -973,12 +975,16 @@ size_t zpass_lds_bytes(int QPB, int nz) {
     return need + (size_t)QPB * 24 * sizeof(double);
 }
 
+static inline int zbatch(const ZInvArgs &a) { return a.bt.nb > 1 ? a.bt.nb : 1; }
+static inline int zbatch(const ZTendArgs &a) { return a.bt.nb > 1 ? a.bt.nb : 1; }
+static inline int zbatch(const ZOpArgs &) { return 1; }
+
 // (ZPT, nz) pairs with a compile-time FFT plan; other nz use the runtime plan.
 #define PS3D_Z_SIZES(X) X(1, 32) X(1, 48) X(1, 64) X(1, 96) X(1, 128) X(1, 192) X(1, 256) X(1, 384) \
     X(2, 512) X(2, 768) X(3, 1024)
 
 #define ZLAUNCH(KERNEL, Z_, N_, ARGS, LAUNCH) \
-    hipLaunchKernelGGL((KERNEL<Z_, N_>), dim3(LAUNCH.blocks), dim3(LAUNCH.threads), LAUNCH.lds, stream, ARGS)
+    hipLaunchKernelGGL((KERNEL<Z_, N_>), dim3(LAUNCH.blocks, zbatch(ARGS)), dim3(LAUNCH.threads), LAUNCH.lds, stream, ARGS)
 
 #define ZDISPATCH_FFT(KERNEL, ARGS, LAUNCH)                                              \
     do {                                                                                 \
@@ -1028,9 +1034,9 @@ hipError_t launch_ztend(const ZLaunch &zl, const ZTendArgs &args, hipStream_t st
 }
 hipError_t launch_zrk(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream) {
     switch (zl.zpt) {
-        case 1: hipLaunchKernelGGL(k_zrk<1>, dim3(zl.blocks), dim3(zl.threads), zl.lds, stream, args); break;
-        case 2: hipLaunchKernelGGL(k_zrk<2>, dim3(zl.blocks), dim3(zl.threads), zl.lds, stream, args); break;
-        case 3: hipLaunchKernelGGL(k_zrk<3>, dim3(zl.blocks), dim3(zl.threads), zl.lds, stream, args); break;
+        case 1: hipLaunchKernelGGL(k_zrk<1>, dim3(zl.blocks, zbatch(args)), dim3(zl.threads), zl.lds, stream, args); break;
+        case 2: hipLaunchKernelGGL(k_zrk<2>, dim3(zl.blocks, zbatch(args)), dim3(zl.threads), zl.lds, stream, args); break;
+        case 3: hipLaunchKernelGGL(k_zrk<3>, dim3(zl.blocks, zbatch(args)), dim3(zl.threads), zl.lds, stream, args); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1042,7 +1048,7 @@ hipError_t launch_zop(const ZLaunch &zl, const ZOpArgs &args, hipStream_t stream
 }
 hipError_t launch_mean(const MeanArgs &args, int nz, hipStream_t stream) {
     size_t lds = 2 * 2 * (size_t)nz * sizeof(double2) + 2 * 256 * sizeof(double);
-    hipLaunchKernelGGL(k_mean, dim3(1), dim3(256), lds, stream, args);
+    hipLaunchKernelGGL(k_mean, dim3(1, args.bt.nb > 1 ? args.bt.nb : 1), dim3(256), lds, stream, args);
     return hipGetLastError();
 }
 

@@ -69,7 +69,7 @@ struct WC {
     int kx, ky;
 };
 
-__device__ inline WC make_wc(const Grid &g, const ZTabs &tb) {
+__device__ inline WC make_wc(const Grid &g, const ZTabs &tb, long ioff) {
     WC c;
     c.lane = threadIdx.x & 63;
     c.m = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -85,7 +85,7 @@ __device__ inline WC make_wc(const Grid &g, const ZTabs &tb) {
     // a member that coincides with member 0 / 1 / 2 is computed but not stored
     c.store = c.m == 0 || (c.m == 1 && dx) || (c.m == 2 && dy) || (c.m == 3 && dx && dy);
     c.colidx = (size_t)ix * g.ny + iy;
-    c.col = c.colidx * (size_t)g.zs;
+    c.col = c.colidx * (size_t)g.zs + ioff;   // ioff: batched instance
     // diffx: ds(k) = si * rk(k) * fs(n-k), si = +1 for k > n/2, -1 otherwise;
     // ds(0) = 0 and ds(n/2) = 0 for even n (sta3dfft.f90:318-335)
     double fxm = (ix >= g.nwx + 1) ? tb.rkx[ix] : -tb.rkx[ix];
@@ -392,7 +392,7 @@ __device__ inline void wrk_update(const WC &c, const RkArgs &rk, int mode, int s
                                   const ZTabs &tb) {
     if (!c.store) return;
     constexpr int NS = WDim<NZ>::S;
-    const double *sc = rk.scal;
+    const double *sc = rk.scal + (size_t)blockIdx.y * SC_NSCAL;
     const double dt = sc[SC_DT];
     double f0;
     if (c.q00)
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(256) k_zinv_w(ZInvArgs a) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     extern __shared__ double2 wsm[];
     const Grid g = a.g;
-    const WC c = make_wc(g, a.tb);
+    const WC c = make_wc(g, a.tb, (long)blockIdx.y * a.bt.vol);
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
     const WTw<P> tw = make_tw<P>(a.twz, c.lane);
     const double sc = a.zscale;
@@ -687,7 +687,7 @@ __global__ void __launch_bounds__(256) k_ztend_w(ZTendArgs a) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     extern __shared__ double2 wsm[];
     const Grid g = a.g;
-    const WC c = make_wc(g, a.tb);
+    const WC c = make_wc(g, a.tb, (long)blockIdx.y * a.bt.vol);
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
     const WTw<P> tw = make_tw<P>(a.twz, c.lane);
     const double sc = a.zscale;
@@ -752,7 +752,7 @@ __global__ void __launch_bounds__(256) k_zop_w(ZOpArgs a) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     extern __shared__ double2 wsm[];
     const Grid g = a.g;
-    const WC c = make_wc(g, a.tb);
+    const WC c = make_wc(g, a.tb, 0);
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
     const WTw<P> tw = make_tw<P>(a.twz, c.lane);
     const double sc = a.zscale;
@@ -834,24 +834,24 @@ hipError_t configure_zwave() {
     return e;
 }
 
-#define ZW_DISPATCH(KERNEL, NQ, ARGS)                                                                   \
+#define ZW_DISPATCH(KERNEL, NQ, NB, ARGS)                                                                   \
     switch (nz) {                                                                                       \
-        case 128: hipLaunchKernelGGL(KERNEL<128>, dim3(NQ), dim3(256), zwave_lds_bytes(128), stream, ARGS); break; \
-        case 256: hipLaunchKernelGGL(KERNEL<256>, dim3(NQ), dim3(256), zwave_lds_bytes(256), stream, ARGS); break; \
-        case 512: hipLaunchKernelGGL(KERNEL<512>, dim3(NQ), dim3(256), zwave_lds_bytes(512), stream, ARGS); break; \
+        case 128: hipLaunchKernelGGL(KERNEL<128>, dim3(NQ, NB), dim3(256), zwave_lds_bytes(128), stream, ARGS); break; \
+        case 256: hipLaunchKernelGGL(KERNEL<256>, dim3(NQ, NB), dim3(256), zwave_lds_bytes(256), stream, ARGS); break; \
+        case 512: hipLaunchKernelGGL(KERNEL<512>, dim3(NQ, NB), dim3(256), zwave_lds_bytes(512), stream, ARGS); break; \
         default: return hipErrorInvalidValue;                                                           \
     }
 
 hipError_t launch_zinv_w(int nz, int nquads, const ZInvArgs &args, hipStream_t stream) {
-    ZW_DISPATCH(k_zinv_w, nquads, args);
+    ZW_DISPATCH(k_zinv_w, nquads, (args.bt.nb > 1 ? args.bt.nb : 1), args);
     return hipGetLastError();
 }
 hipError_t launch_ztend_w(int nz, int nquads, const ZTendArgs &args, hipStream_t stream) {
-    ZW_DISPATCH(k_ztend_w, nquads, args);
+    ZW_DISPATCH(k_ztend_w, nquads, (args.bt.nb > 1 ? args.bt.nb : 1), args);
     return hipGetLastError();
 }
 hipError_t launch_zop_w(int nz, int nquads, const ZOpArgs &args, hipStream_t stream) {
-    ZW_DISPATCH(k_zop_w, nquads, args);
+    ZW_DISPATCH(k_zop_w, nquads, 1, args);
     return hipGetLastError();
 }
 

@@ -63,6 +63,17 @@ def lib():
         L.ps3d_gpu_rk_substep.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
         L.ps3d_gpu_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.ps3d_gpu_profile_kernels.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ps3d_gpu_create_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.ps3d_gpu_batch_size.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.ps3d_gpu_set_vorticity_batch.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ps3d_gpu_get_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.ps3d_gpu_advance_n_batch.argtypes = [C.c_void_p, _dp, C.c_int, C.c_void_p]
+        L.ps3d_gpu_ensemble_create.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
+        L.ps3d_gpu_ensemble_group.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.ps3d_gpu_ensemble_advance.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.ps3d_gpu_ensemble_tune.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.ps3d_gpu_ensemble_set_choice.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ps3d_gpu_ensemble_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -218,6 +229,107 @@ class Ps3dGpu:
 
     def field_decompose_physical(self, f):
         return self.op(OP_DECOMPOSE_PHYS, f)
+
+
+class Ps3dBatch:
+    """nbatch instances of one grid stepped by the same kernel launches (a batched
+    ps3d_handle).  Owned (closed) unless it belongs to an ensemble."""
+
+    def __init__(self, params=None, nbatch=1, _handle=None):
+        if _handle is not None:
+            self.h, self.owned = _handle, False
+            nb = C.c_int()
+            _check(lib().ps3d_gpu_batch_size(self.h, C.byref(nb)))
+            self.nbatch = nb.value
+        else:
+            if not isinstance(params, Params):
+                params = _convert(params)
+            self.h, self.owned = C.c_void_p(), True
+            _check(lib().ps3d_gpu_create_batch(C.byref(params), int(nbatch), C.byref(self.h)))
+            self.nbatch = int(nbatch)
+            self.set_shape(params.nx, params.ny, params.nz)
+        self.p = params
+        self.t = np.zeros(self.nbatch)
+
+    def set_shape(self, nx, ny, nz):
+        self.shape = (nx, ny, nz + 1)
+
+    def set_vorticity(self, b, vor):
+        vor = np.ascontiguousarray(vor, dtype=np.float64)
+        _check(lib().ps3d_gpu_set_vorticity_batch(self.h, int(b), vor))
+        self.t[b] = 0.0
+
+    def get_svor(self, b):
+        out = np.zeros((3,) + self.shape)
+        _check(lib().ps3d_gpu_get_batch(self.h, int(b), SVOR, out))
+        return out
+
+    def advance_n(self, nsteps, series=False):
+        diag = np.zeros((self.nbatch, max(1, nsteps), NDIAG)) if series else None
+        tt = np.ascontiguousarray(self.t, dtype=np.float64)
+        _check(lib().ps3d_gpu_advance_n_batch(self.h, tt, int(nsteps),
+                                              diag.ctypes.data if series else None))
+        self.t = tt
+        return diag[:, :nsteps] if series else None
+
+    def close(self):
+        if self.owned and self.h:
+            lib().ps3d_gpu_destroy(self.h)
+        self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Ps3dEnsemble:
+    """Several grid sizes, each a batched group, stepped concurrently; one step of
+    every instance can replay as one captured HIP graph."""
+
+    def __init__(self, params_list, counts):
+        n = len(params_list)
+        arr = (Params * n)()
+        for i, p in enumerate(params_list):
+            arr[i] = p if isinstance(p, Params) else _convert(p)
+        cnt = (C.c_int * n)(*[int(c) for c in counts])
+        self.e = C.c_void_p()
+        _check(lib().ps3d_gpu_ensemble_create(n, arr, cnt, C.byref(self.e)))
+        self.groups = []
+        for g in range(n):
+            h = C.c_void_p()
+            _check(lib().ps3d_gpu_ensemble_group(self.e, g, C.byref(h)))
+            b = Ps3dBatch(_handle=h)
+            b.p = arr[g]
+            b.set_shape(arr[g].nx, arr[g].ny, arr[g].nz)
+            self.groups.append(b)
+
+    def advance(self, nsteps, graph=True):
+        ms = C.c_double()
+        _check(lib().ps3d_gpu_ensemble_advance(self.e, int(nsteps), 1 if graph else 0, C.byref(ms)))
+        return ms.value
+
+    def tune(self, steps=2):
+        ch = (C.c_int * len(self.groups))()
+        _check(lib().ps3d_gpu_ensemble_tune(self.e, int(steps), ch))
+        return list(ch)
+
+    def set_choice(self, g, code):
+        _check(lib().ps3d_gpu_ensemble_set_choice(self.e, int(g), int(code)))
+
+    def close(self):
+        if self.e:
+            for b in self.groups:
+                b.h = C.c_void_p()
+            lib().ps3d_gpu_ensemble_destroy(self.e)
+            self.e = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _convert(p):
