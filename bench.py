@@ -37,6 +37,12 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--ic", choices=["beltrami", "random"], default="beltrami",
+                    help="beltrami: 4 seeded Beltrami modes + noise; random: random-phase isotropic "
+                         "E(k) ~ k^4 exp(-2 (k/4)^2) made on 128^3 and zero-padded to n^3 on the GPU")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-instance ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--separate-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy (y = x + 0) of MIB mebibytes of FP64, 5 times, to "
                          "calibrate the FETCH_SIZE / WRITE_SIZE counters")
@@ -78,6 +84,75 @@ def read_traffic(n):
     return out
 
 
+ENSEMBLE_SIZES = [32, 48, 64, 96]
+ENSEMBLE_COUNTS = [38, 29, 19, 10]     # 40/30/20/10 % of 96 by largest remainder
+
+
+def ensemble_bench(G, steps, sep_steps):
+    """96 instances of four grid sizes, random-phase isotropic ICs with k0 ~ U[2, 6]
+    (seeded), the plain-workload physics.  One batched handle per size, groups on
+    forked streams, one captured graph per step, z launch shapes tuned per size;
+    against the same 96 instances as separate handles stepped one after another."""
+    import numpy as np
+    from ps3d_amd import ics
+    rng = np.random.default_rng(2024)
+    ntot = sum(ENSEMBLE_COUNTS)
+    k0s = rng.uniform(2.0, 6.0, size=ntot)
+    params = [workload_params(G, n) for n in ENSEMBLE_SIZES]
+    ens = G.Ps3dEnsemble(params, ENSEMBLE_COUNTS)
+    vors = []
+    i = 0
+    for gi, n in enumerate(ENSEMBLE_SIZES):
+        for b in range(ENSEMBLE_COUNTS[gi]):
+            v = ics.random_phase_isotropic(n, n, n, k0=float(k0s[i]), seed=1000 + i)
+            ens.groups[gi].set_vorticity(b, v)
+            vors.append((gi, v))
+            i += 1
+    shapes = ens.tune(steps=2)
+    ens.advance(2)                      # builds the graph
+    t0 = time.perf_counter()
+    ev_ms = ens.advance(steps, graph=True)
+    el = time.perf_counter() - t0
+    npts = sum(c * n * n * (n + 1) for n, c in zip(ENSEMBLE_SIZES, ENSEMBLE_COUNTS))
+    ens.close()
+    sep = []
+    for gi, v in vors:
+        h = G.Ps3dGpu(params[gi])
+        h.set_vorticity(v)
+        sep.append(h)
+    for h in sep:
+        h.advance_n(1)
+    t0 = time.perf_counter()
+    for h in sep:
+        h.advance_n(sep_steps)
+    el_sep = time.perf_counter() - t0
+    for h in sep:
+        h.close()
+    b_step = B_STEP_VOLUMES * 8.0 * npts
+    return {
+        "instances": ntot, "sizes": ENSEMBLE_SIZES, "counts": ENSEMBLE_COUNTS, "npts_total": npts,
+        "ic": "random-phase isotropic, k0 ~ U[2,6] per instance (numpy PCG64 seed 2024), IC seeds 1000+i",
+        "steps": steps, "ms_per_step": 1e3 * el / steps, "event_ms_per_step": ev_ms / steps,
+        "grid_point_updates_per_s": npts * steps / el,
+        "z_shapes": {str(n): c for n, c in zip(ENSEMBLE_SIZES, shapes)},
+        "step_roofline": {"bound": "hbm", "algo_bytes_per_step": b_step, "achieved": b_step * steps / el / 1e9,
+                          "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": b_step * steps / el / PEAK_HBM},
+        "separate": {"steps": sep_steps, "ms_per_step": 1e3 * el_sep / sep_steps,
+                     "grid_point_updates_per_s": npts * sep_steps / el_sep},
+        "speedup_vs_separate": (npts * steps / el) / (npts * sep_steps / el_sep),
+    }
+
+
+def initial_vorticity(G, ics, n, kind):
+    if kind == "beltrami":
+        return ics.beltrami_multimode(n, n, n, seed=1), None
+    # random-phase isotropic: band-limited (k0 = 4), exact on 128^3, zero-padded to n^3
+    nc = min(n, 128)
+    coarse = G.Ps3dGpu(workload_params(G, nc))
+    coarse.set_vorticity(ics.random_phase_isotropic(nc, nc, nc, k0=4.0, seed=1))
+    return None, coarse
+
+
 def cpu_baseline(vor, n, steps, threads):
     from oracle import oracle as O
     ncores = threads if threads > 0 else (os.cpu_count() or 1)
@@ -105,9 +180,17 @@ def main():
     from ps3d_amd import ics
 
     n = a.n
-    vor = ics.beltrami_multimode(n, n, n, seed=1)
+    vor, coarse = initial_vorticity(G, ics, n, a.ic)
     d = G.Ps3dGpu(workload_params(G, n))
-    d.set_vorticity(vor)
+    if coarse is None:
+        d.set_vorticity(vor)
+    else:
+        G.resample(d, coarse)
+        coarse.close()
+        if n > 256:
+            a.no_cpu_baseline = True     # the host oracle does not fit this grid in a bounded sample
+        elif not a.no_cpu_baseline:
+            vor = d.get(G.VOR)           # the oracle starts from the same physical vorticity
     if a.warmup:
         d.advance_n(a.warmup)
     d.sync()
@@ -157,8 +240,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain FP64 Beltrami %d^3, impl-diff-rk4, 2/3-rule, adaptive CFL dt" % n,
-                   "grid": [n, n, n], "npts": npts, "ic": "4 Beltrami modes, seeded phases, 1e-3 noise (seed 1)"},
+        "config": {"workload": "plain FP64 %s %d^3, impl-diff-rk4, 2/3-rule, adaptive CFL dt"
+                               % ("Beltrami" if a.ic == "beltrami" else "random-phase isotropic", n),
+                   "grid": [n, n, n], "npts": npts,
+                   "ic": ("4 Beltrami modes, seeded phases, 1e-3 noise (seed 1)" if a.ic == "beltrami" else
+                          "random-phase isotropic E(k)~k^4 exp(-2(k/4)^2), seed 1, made on 128^3, zero-padded")},
         "grid_point_updates_per_s": npts * steps_per_s,
         "event_ms_per_step": ev_ms / a.steps,
         "step_roofline": {"bound": "hbm", "algo_bytes_per_step": b_step,
@@ -169,6 +255,11 @@ def main():
         "t_end": float(diag[-1][0] + diag[-1][1]) if len(diag) else None,
     }
     d.close()
+    if not a.no_ensemble:
+        try:
+            out["ensemble"] = ensemble_bench(G, a.ensemble_steps, a.separate_steps)
+        except Exception as exc:       # reported, never silently dropped
+            out["ensemble"] = {"error": repr(exc)}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vor, n, a.cpu_steps, a.cpu_threads)
     print(json.dumps(out))

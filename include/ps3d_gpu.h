@@ -127,6 +127,15 @@ int ps3d_gpu_destroy(ps3d_handle *h);
  * the kinetic energy and enstrophy (init_diffusion). */
 int ps3d_gpu_set_vorticity(ps3d_handle *h, const double *vor);
 
+/* Spectral resampling between two grids of the same box (no counterpart in the
+ * reference).  which = PS3D_SVOR: dst's initial state is src's vorticity, truncated
+ * to a coarser grid (the sharp spectral cutoff filter) or zero-padded to a finer one,
+ * then set up as ps3d_gpu_set_vorticity does; which = PS3D_SVORTS: the stored
+ * tendency, copied the same way.  Horizontal stafft coefficients scale by
+ * sqrt(n_dst/n_src) per axis, interior sine coefficients also by
+ * sqrt(nz_dst/nz_src); Nyquist slots of either grid are dropped. */
+int ps3d_gpu_resample(ps3d_handle *dst, ps3d_handle *src, int which);
+
 /* = vor2vel (src/inversion/inversion.f90:23-226) on the device state. */
 int ps3d_gpu_vor2vel(ps3d_handle *h);
 /* = source -> vorticity_tendency (src/inversion/inversion.f90:298-388). */

@@ -136,6 +136,17 @@ struct PresArgs {
     double *pres;
 };
 hipError_t launch_pres_rhs(const PresArgs &a, hipStream_t stream);
+
+// Spectral resampling of a mixed-representation field between grids of the same
+// box: truncation (to a coarser grid) or zero padding (to a finer one).
+struct ResampleArgs {
+    const double *src[3];
+    double *dst[3];
+    int snx, sny, snz, szs;
+    int dnx, dny, dnz, dzs;
+    double hfac, zfac;
+};
+hipError_t launch_resample(const ResampleArgs &a, hipStream_t stream);
 hipError_t launch_add2(long n, const double *a, const double *b, double *out, hipStream_t stream);
 
 // ---------------------------------------------------------------------------

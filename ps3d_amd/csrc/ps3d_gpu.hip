@@ -745,6 +745,8 @@ int ps3d_gpu_sync(ps3d_handle *h) {
 }
 
 // setup_fields (utils.f90:136-184)
+static int finish_setup(ps3d_handle *h);
+
 int ps3d_gpu_set_vorticity(ps3d_handle *h, const double *vor) {
     if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !vor) return fail(PS3D_E_ARG, "null argument");
@@ -756,6 +758,13 @@ int ps3d_gpu_set_vorticity(ps3d_handle *h, const double *vor) {
     if ((st = line_fields(h, 1, LM_FWD, 3, h->vel, h->vel))) return st;
     if ((st = line_fields(h, 0, LM_FWD, 3, h->vel, h->vel))) return st;
     for (int c = 0; c < 3; ++c) HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DECOMPOSE, h->vel[c], h->svor[c]), h->stream));
+    return finish_setup(h);
+}
+
+// The part of setup_fields after svor holds the initial vorticity: ini_vor_mean,
+// t = 0, rolling-mean state, vor2vel and init_diffusion (utils.f90:160-184).
+static int finish_setup(ps3d_handle *h) {
+    int st;
     if ((st = run_mean(h, MEAN_CALC, 0))) return st;
     HIPCHK(hipMemsetAsync(h->scal + SC_T, 0, sizeof(double), h->stream));
     double init[4] = {0.0, 1.0, 1.0, 0.0};   // rolling mean: sma, inew, iold, filled
@@ -1171,6 +1180,38 @@ int ps3d_gpu_fft1d(int kind, int m, int n, double *x) {
     return st;
 }
 
+
+int ps3d_gpu_resample(ps3d_handle *dst, ps3d_handle *src, int which) {
+    if (!dst || !src) return fail(PS3D_E_ARG, "null handle");
+    if (dst->nbatch != 1 || src->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: resample one instance at a time");
+    if (which != PS3D_SVOR && which != PS3D_SVORTS) return fail(PS3D_E_UNSUPPORTED, "resample: svor or svorts only");
+    if (which == PS3D_SVOR && !src->have_state) return fail(PS3D_E_STATE, "resample of svor before set_vorticity");
+    for (int d = 0; d < 3; ++d)
+        if (dst->p.extent[d] != src->p.extent[d] || dst->p.lower[d] != src->p.lower[d])
+            return fail(PS3D_E_ARG, "resample needs the same box");
+    HIPCHK(hipStreamSynchronize(src->stream));
+    ResampleArgs a;
+    std::memset(&a, 0, sizeof a);
+    double *const *s = which == PS3D_SVOR ? src->svor : src->mid;
+    double *const *d = which == PS3D_SVOR ? dst->svor : dst->mid;
+    for (int c = 0; c < 3; ++c) {
+        a.src[c] = s[c];
+        a.dst[c] = d[c];
+    }
+    a.snx = src->g.nx; a.sny = src->g.ny; a.snz = src->g.nz; a.szs = src->g.zs;
+    a.dnx = dst->g.nx; a.dny = dst->g.ny; a.dnz = dst->g.nz; a.dzs = dst->g.zs;
+    // stafft coefficients scale as sqrt(n) per axis (1/sqrt(n) forward transform);
+    // the orthonormal DST-I interior as sqrt(nz)
+    a.hfac = std::sqrt(((double)a.dnx * a.dny) / ((double)a.snx * a.sny));
+    a.zfac = std::sqrt((double)a.dnz / (double)a.snz);
+    HIPCHK(launch_resample(a, dst->stream));
+    if (which == PS3D_SVORTS) {
+        HIPCHK(hipStreamSynchronize(dst->stream));
+        return 0;
+    }
+    dst->have_state = true;
+    return finish_setup(dst);
+}
 
 // ---------------------------------------------------------------------------
 // Batched instances of one grid, and ensembles of several grids

@@ -887,6 +887,49 @@ __global__ void __launch_bounds__(256) k_add2(long n, const double *__restrict__
         out[i] = x[i] + y[i];
 }
 
+// Horizontal slot map of the stafft Hermitian form (cos slot k <= n/2, sine slot
+// n-k of wavenumber k) from a destination index to the source index of the same
+// wavenumber, or -1.  The Nyquist slot of either grid is left out (zero).
+__device__ inline int slot_map(int i, int nd, int ns) {
+    if (i == 0) return 0;
+    const int hd = nd / 2, hs = ns / 2;
+    if (i < hd) return i < hs ? i : -1;
+    if (i == hd) return -1;
+    const int m = nd - i;
+    return m < hs ? ns - m : -1;
+}
+
+// One destination column per block: lids scaled by hfac, interior sine
+// coefficients kz < min(nz) by hfac * zfac, the rest zero.
+__global__ void __launch_bounds__(256) k_resample(ResampleArgs a) {
+    const int col = blockIdx.x;
+    const int ix = col / a.dny, iy = col % a.dny;
+    const int sx = slot_map(ix, a.dnx, a.snx), sy = slot_map(iy, a.dny, a.sny);
+    const bool have = sx >= 0 && sy >= 0;
+    const size_t dcol = (size_t)col * a.dzs;
+    const size_t scol = have ? ((size_t)sx * a.sny + sy) * a.szs : 0;
+    for (int z = threadIdx.x; z <= a.dnz; z += blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double v = 0.0;
+            if (have) {
+                if (z == 0)
+                    v = a.src[c][scol] * a.hfac;
+                else if (z == a.dnz)
+                    v = a.src[c][scol + a.snz] * a.hfac;
+                else if (z < a.snz)
+                    v = a.src[c][scol + z] * (a.hfac * a.zfac);
+            }
+            a.dst[c][dcol + z] = v;
+        }
+    }
+}
+
+hipError_t launch_resample(const ResampleArgs &a, hipStream_t stream) {
+    hipLaunchKernelGGL(k_resample, dim3(a.dnx * a.dny), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
 static unsigned pointwise_blocks(long n) {
     const long b = (n + 255) / 256;
     return (unsigned)std::min<long>(b, 256L * 64L);

@@ -63,6 +63,7 @@ def lib():
         L.ps3d_gpu_rk_substep.argtypes = [C.c_void_p, C.c_int, C.c_double, _dp]
         L.ps3d_gpu_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.ps3d_gpu_profile_kernels.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ps3d_gpu_resample.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.ps3d_gpu_create_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.ps3d_gpu_batch_size.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.ps3d_gpu_set_vorticity_batch.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -84,6 +85,14 @@ def _check(st):
 
 
 FFT1D_KIND = {"forfft": 0, "revfft": 1, "dct": 2, "dst": 3}
+
+
+def resample(dst, src, which=SVOR):
+    """Spectral truncation / zero padding of src's svor (or svorts) onto dst's grid
+    (same box); with SVOR, dst is then set up as by set_vorticity."""
+    _check(lib().ps3d_gpu_resample(dst.h, src.h, int(which)))
+    if which == SVOR:
+        dst.t = 0.0
 
 
 def fft1d(kind, x):
