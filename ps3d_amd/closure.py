@@ -1,0 +1,213 @@
+"""Learned subgrid closure for a coarse grid: the 3x3x3 convolution net of
+ps3d_closure.h (bf16 matrix cores), its training data from filtered fine-grid
+runs, training, and the genspec-style energy spectrum used to judge it.
+
+Training target: tau = R(S_f) - S_c(R(omega_f)), where S is ps3d's vorticity
+tendency (src/inversion/inversion.f90:298-388), R the sharp spectral truncation
+of a fine grid onto the coarse one (ps3d_gpu_resample) and omega_f a fine-grid
+state; input: the coarse u, v, w, xi, eta, zeta.  Both are physical fields on the
+coarse points, scaled to unit rms per group (velocity, vorticity, tendency).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import gpu as G
+from .gpu import _check as _gcheck
+from .gpu import lib as _glib
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = _glib()
+        L.ps3d_cnn_last_error.restype = C.c_char_p
+        L.ps3d_cnn_nparams.argtypes = [C.POINTER(C.c_int)]
+        L.ps3d_cnn_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.ps3d_cnn_destroy.argtypes = [C.c_void_p]
+        fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.ps3d_cnn_set_weights.argtypes = [C.c_void_p, fp]
+        L.ps3d_cnn_get_weights.argtypes = [C.c_void_p, fp]
+        L.ps3d_cnn_forward.argtypes = [C.c_void_p, fp, fp]
+        L.ps3d_cnn_grad.argtypes = [C.c_void_p, fp, fp, C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        L.ps3d_cnn_train_step.argtypes = [C.c_void_p, fp, fp, C.c_float, C.POINTER(C.c_float)]
+        L.ps3d_cnn_dataset_reserve.argtypes = [C.c_void_p, C.c_int]
+        L.ps3d_cnn_dataset_set.argtypes = [C.c_void_p, C.c_int, fp, fp]
+        L.ps3d_cnn_train_epoch.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+                                           C.c_int, C.c_float, C.POINTER(C.c_float)]
+        L.ps3d_cnn_timing.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.ps3d_gpu_set_closure.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        raise G.Ps3dError(st, lib().ps3d_cnn_last_error().decode())
+
+
+LAYERS = [(6, 32), (32, 32), (32, 32), (32, 3)]
+GFLOP_FWD_PER_POINT = 2.0 * 27 * sum(ci * co for ci, co in LAYERS) / 1e9   # algorithmic (unpadded)
+
+
+def nparams():
+    n = C.c_int()
+    _check(lib().ps3d_cnn_nparams(C.byref(n)))
+    return n.value
+
+
+def init_weights(seed=0):
+    """Same initialisation as the oracle: W ~ U(-s, s), s = sqrt(6/(27 cin)), b = 0
+    (the last layer is scaled by 0.1 so the untrained closure is small)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for li, (cin, cout) in enumerate(LAYERS):
+        s = math.sqrt(6.0 / (27 * cin)) * (0.1 if li == len(LAYERS) - 1 else 1.0)
+        out.append(rng.uniform(-s, s, size=27 * cout * cin))
+        out.append(np.zeros(cout))
+    return np.concatenate(out).astype(np.float32)
+
+
+class ClosureNet:
+    def __init__(self, nx, ny, nz):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.npts = nx * ny * (nz + 1)
+        self.h = C.c_void_p()
+        _check(lib().ps3d_cnn_create(nx, ny, nz, C.byref(self.h)))
+        self.in_scale = np.ones(6, dtype=np.float32)
+        self.out_scale = np.ones(3, dtype=np.float32)
+
+    def close(self):
+        if self.h:
+            lib().ps3d_cnn_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_weights(self, w):
+        _check(lib().ps3d_cnn_set_weights(self.h, np.ascontiguousarray(w, dtype=np.float32)))
+
+    def get_weights(self):
+        w = np.zeros(nparams(), dtype=np.float32)
+        _check(lib().ps3d_cnn_get_weights(self.h, w))
+        return w
+
+    def forward(self, x):
+        y = np.zeros((self.npts, 3), dtype=np.float32)
+        _check(lib().ps3d_cnn_forward(self.h, np.ascontiguousarray(x, dtype=np.float32).reshape(self.npts, 6), y))
+        return y
+
+    def grad(self, x, t, want_gx=False):
+        loss = C.c_float()
+        g = np.zeros(nparams(), dtype=np.float32)
+        gx = np.zeros((self.npts, 6), dtype=np.float32) if want_gx else None
+        _check(lib().ps3d_cnn_grad(self.h, np.ascontiguousarray(x, dtype=np.float32).reshape(self.npts, 6),
+                                   np.ascontiguousarray(t, dtype=np.float32).reshape(self.npts, 3), C.byref(loss),
+                                   g.ctypes.data, gx.ctypes.data if want_gx else None))
+        return loss.value, g, gx
+
+    def train_step(self, x, t, lr):
+        loss = C.c_float()
+        _check(lib().ps3d_cnn_train_step(self.h, np.ascontiguousarray(x, dtype=np.float32).reshape(self.npts, 6),
+                                         np.ascontiguousarray(t, dtype=np.float32).reshape(self.npts, 3),
+                                         float(lr), C.byref(loss)))
+        return loss.value
+
+    def reserve(self, n):
+        _check(lib().ps3d_cnn_dataset_reserve(self.h, int(n)))
+
+    def set_sample(self, i, x, t):
+        _check(lib().ps3d_cnn_dataset_set(self.h, int(i), np.ascontiguousarray(x, dtype=np.float32).reshape(self.npts, 6),
+                                          np.ascontiguousarray(t, dtype=np.float32).reshape(self.npts, 3)))
+
+    def train_epoch(self, order, lr):
+        loss = C.c_float()
+        order = np.ascontiguousarray(order, dtype=np.int32)
+        _check(lib().ps3d_cnn_train_epoch(self.h, order, len(order), float(lr), C.byref(loss)))
+        return loss.value
+
+    def timing(self, reps=20):
+        f, t = C.c_float(), C.c_float()
+        _check(lib().ps3d_cnn_timing(self.h, int(reps), C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def attach(self, solver):
+        """Add this net's tendency correction to every RK stage of solver.advance_n."""
+        ins = np.ascontiguousarray(self.in_scale, dtype=np.float32)
+        outs = np.ascontiguousarray(self.out_scale, dtype=np.float32)
+        self._keep = (ins, outs)
+        _gcheck(lib().ps3d_gpu_set_closure(solver.h, self.h, ins.ctypes.data, outs.ctypes.data))
+
+    @staticmethod
+    def detach(solver):
+        _gcheck(lib().ps3d_gpu_set_closure(solver.h, None, None, None))
+
+
+# --------------------------------------------------------------------------
+# data: channels-last point arrays from solver fields
+# --------------------------------------------------------------------------
+def to_points(f):
+    """(nc, nx, ny, nz+1) solver-layout components -> (npts, nc), p = (ix*ny+iy)*(nz+1)+iz."""
+    return np.ascontiguousarray(np.moveaxis(f, 0, -1).reshape(-1, f.shape[0]))
+
+
+def sample_from_fine(fine, coarse, tcoarse):
+    """One training pair at the current state of `fine` (its vor2vel and source
+    run here): input = coarse (u, v, w, omega) of R(omega_f); target = R(S_f) -
+    S_c(R(omega_f)) in physical space on the coarse points."""
+    fine.vor2vel()
+    fine.source()
+    G.resample(coarse, fine)                 # coarse state = truncated vorticity (set up, vor2vel)
+    vel = coarse.get(G.VEL)
+    vor = coarse.get(G.VOR)
+    coarse.source()
+    sc = coarse.get(G.SVORTS)
+    G.resample(tcoarse, fine, G.SVORTS)      # R(S_f)
+    sf = tcoarse.get(G.SVORTS)
+    tau = np.stack([coarse.op(G.OP_COMBINE_PHYS, sf[c] - sc[c]) for c in range(3)])
+    x = to_points(np.concatenate([vel, vor]))
+    return x.astype(np.float32), to_points(tau).astype(np.float32)
+
+
+# --------------------------------------------------------------------------
+# genspec energy spectrum (src/genspec.f90:50-124)
+# --------------------------------------------------------------------------
+def energy_spectrum(solver, ke):
+    """Spectrum of the solver's current physical velocity (after vor2vel) in
+    shells of K = sqrt(kx^2 + ky^2 + kz^2), normalised so sum(spec) dk = ke."""
+    vel = solver.get(G.VEL)
+    nx, ny, nz = solver.nx, solver.ny, solver.nz
+    s = [solver.op(G.OP_FFTXYP2S, vel[c]) for c in range(3)]
+    s[0] = solver.op(G.OP_FFTCOSINE, s[0])
+    s[1] = solver.op(G.OP_FFTCOSINE, s[1])
+    # dst(1, nz, svel(1:nz)): w(1..nz-1) sine coefficients, w(nz) set to 0, w(0) kept
+    s[2] = solver.op(G.OP_FFTSINE, s[2])
+    lx, ly, lz = solver.p.extent[0], solver.p.extent[1], solver.p.extent[2]
+    kx = 2 * np.pi / lx * np.minimum(np.arange(nx), nx - np.arange(nx))
+    ky = 2 * np.pi / ly * np.minimum(np.arange(ny), ny - np.arange(ny))
+    kz = np.pi / lz * np.arange(nz + 1)
+    kmag = np.rint(np.sqrt(kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2)).astype(int)
+    kmax = int(kmag.max())
+    dk = kmax / math.sqrt((0.5 * nx) ** 2 + (0.5 * ny) ** 2 + float(nz) ** 2)
+    m = (kmag * (1.0 / dk)).astype(int)
+    e = s[0] ** 2 + s[1] ** 2 + s[2] ** 2
+    spec = np.bincount(m.ravel(), weights=e.ravel(), minlength=kmax + 1)[:kmax + 1]
+    num = np.bincount(m.ravel(), minlength=kmax + 1)[:kmax + 1]
+    pref = 4.0 / 3.0 * np.pi * dk ** 3
+    ok = num > 0
+    spec[ok] = spec[ok] * pref * ((np.arange(kmax + 1)[ok] + 1) ** 3 - np.arange(kmax + 1)[ok] ** 3) / num[ok]
+    spec *= ke / np.sum(spec * dk)
+    return np.arange(kmax + 1) * dk, spec
+
+
+def spectrum_error(spec, ref, kcut=None):
+    """Relative L2 difference of two spectra over the shells below kcut."""
+    n = min(len(spec), len(ref)) if kcut is None else kcut
+    return float(np.linalg.norm(spec[:n] - ref[:n]) / np.linalg.norm(ref[:n]))

@@ -1,6 +1,10 @@
 // kernels.hpp -- kernel argument blocks and launchers shared by the passes.
 #pragma once
+#include <cstdint>
+
 #include "common.hpp"
+
+struct ps3d_cnn;
 
 namespace ps3d {
 
@@ -88,6 +92,7 @@ struct ZTendArgs {
     int mode, stage;
     RkArgs rk;
     Batch bt;
+    const double *extra[3];   // optional mixed-spectral tendency added to the source (closure)
 };
 
 struct ZOpArgs {
@@ -187,6 +192,8 @@ struct LineArgs {
     double ncelli;
     double sqrt_nx;    // for means from the kx = 0 slot
     Batch bt;
+    uint16_t *nn_in;   // LM_XMAIN: also write the closure input (bf16 [p][16]) when set
+    float nn_scale[6];
 };
 
 size_t line_lds_bytes(int n, int ZC);
@@ -210,5 +217,10 @@ struct FinArgs {
 };
 hipError_t launch_fin_y(const FinArgs &a, hipStream_t stream);
 hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream);
+
+// closure hooks (closure.hip)
+bool cnn_matches(const ps3d_cnn *c, int nx, int ny, int nz);
+uint16_t *cnn_input(ps3d_cnn *c);
+hipError_t cnn_apply(ps3d_cnn *c, const float *out_scale, int zs, double *o0, double *o1, double *o2, hipStream_t s);
 
 }  // namespace ps3d

@@ -210,6 +210,24 @@ __device__ inline void fwd_store(const Tile &t, const LineArgs &a, double2 *A, d
     __syncthreads();
 }
 
+// closure input: channel ch of the bf16 tensor [p][16], p = (ix*ny + iy)*(nz+1) + iz,
+// from a physical field in registers (lines along x)
+__device__ inline void nn_export(const Tile &t, const LineArgs &a, const double2 (&r)[kEpt], int ch) {
+    if (!a.nn_in) return;
+    const float s = a.nn_scale[ch];
+#pragma unroll
+    for (int k = 0; k < kEpt; ++k)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (!reg_valid(t, k, c, a.g.nz)) continue;
+            const long p = ((long)reg_j(t, k) * a.g.ny + t.other) * (a.g.nz + 1) + reg_z(t, k, c);
+            const float v = (float)(c ? r[k].y : r[k].x) * s;
+            uint32_t u = __builtin_bit_cast(uint32_t, v);
+            u += 0x7fffu + ((u >> 16) & 1u);
+            a.nn_in[p * 16 + ch] = (uint16_t)(u >> 16);
+        }
+}
+
 // block-wide reductions (deterministic tree in LDS)
 __device__ inline double block_sum(double v, double *scratch) {
     const int tid = threadIdx.x;
@@ -455,8 +473,12 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         field_phys<N>(t, a, a.in[0], A, B, u, false);
         field_phys<N>(t, a, a.in[1], A, B, v, false);
         field_phys<N>(t, a, a.in[2], A, B, w, false);
+        nn_export(t, a, u, 0);
+        nn_export(t, a, v, 1);
+        nn_export(t, a, w, 2);
         double2 o[kEpt];
         field_phys<N>(t, a, a.in[3], A, B, o, false);   // xi
+        nn_export(t, a, o, 3);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
             const double xr = o[k].x + a.fcor[0], xi = o[k].y + a.fcor[0];
@@ -464,6 +486,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
             q[k] = make_double2(w[k].x * xr, w[k].y * xi);
         }
         field_phys<N>(t, a, a.in[4], A, B, o, false);   // eta
+        nn_export(t, a, o, 4);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
             const double er = o[k].x + a.fcor[1], ei = o[k].y + a.fcor[1];
@@ -471,6 +494,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
             p[k] = make_double2(-w[k].x * er, -w[k].y * ei);
         }
         field_phys<N>(t, a, a.in[5], A, B, o, false);   // zeta
+        nn_export(t, a, o, 5);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
             const double zr = o[k].x + a.fcor[2], zi = o[k].y + a.fcor[2];

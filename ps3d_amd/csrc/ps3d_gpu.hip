@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/ps3d_closure.h"
 #include "../../include/ps3d_gpu.h"
 #include "kernels.hpp"
 
@@ -70,6 +71,8 @@ struct ps3d_handle {
     int nbatch = 1;
     long b_vol = 0, b_party = 0, b_partx = 0, b_roll = 0, b_diag = 0;
     std::vector<char> inst_set;    // instances given an initial vorticity
+    ps3d_cnn *cnn = nullptr;       // attached closure (owned by the caller)
+    float cnn_in[6]{}, cnn_out[3]{};
     bool have_grad = false;   // grad holds the physical y-derivatives of u, v, w (vor2vel, before source)
     bool have_derived = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -569,6 +572,10 @@ static int products(ps3d_handle *h, int phys_in, double *const *dst) {
         x.in[3 + c] = h->vor[c];
         x.out[c] = dst[c];
     }
+    if (h->cnn) {   // the x pass also writes the closure input
+        x.nn_in = ps3d::cnn_input(h->cnn);
+        for (int k = 0; k < 6; ++k) x.nn_scale[k] = h->cnn_in[k];
+    }
     prof_begin(h, PS3D_K_XMAIN);
     int st = run_line(h, x);
     prof_end(h);
@@ -599,6 +606,17 @@ static int fused_stage(ps3d_handle *h, int mode, int stage) {
     }
     if ((st = products(h, 0, h->vel))) return st;
     ZTendArgs t = ztend_args(h, mode, stage, h->vel);
+    if (h->cnn) {
+        // closure: net on (u, v, w, omega) of this stage -> physical tendency in grad,
+        // decomposed like a physical field into mid, added to the source in the z pass
+        HIPCHK(ps3d::cnn_apply(h->cnn, h->cnn_out, h->g.zs, h->grad[0], h->grad[1], h->grad[2], h->stream));
+        if ((st = line_fields(h, 1, LM_FWD, 3, h->grad, h->grad))) return st;
+        if ((st = line_fields(h, 0, LM_FWD, 3, h->grad, h->grad))) return st;
+        for (int c = 0; c < 3; ++c) {
+            HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DECOMPOSE, h->grad[c], h->mid[c]), h->stream));
+            t.extra[c] = h->mid[c];
+        }
+    }
     prof_begin(h, PS3D_K_ZTEND);
     HIPCHK(launch_ztend(h->zl, t, h->stream));
     prof_end(h);
@@ -1507,6 +1525,22 @@ int ps3d_gpu_ensemble_set_choice(ps3d_ensemble *e, int g, int code) {
     if (std::find(cand.begin(), cand.end(), code) == cand.end()) return fail(PS3D_E_ARG, "launch shape not available");
     apply_z_choice(h, code);
     drop_graph(e);
+    return 0;
+}
+
+
+int ps3d_gpu_set_closure(ps3d_handle *h, ps3d_cnn *net, const float *in_scale, const float *out_scale) {
+    if (!h) return fail(PS3D_E_ARG, "null handle");
+    if (!net) {
+        h->cnn = nullptr;
+        return 0;
+    }
+    if (!in_scale || !out_scale) return fail(PS3D_E_ARG, "null scales");
+    if (h->nbatch != 1) return fail(PS3D_E_STATE, "closure on batched handles is not supported");
+    if (!ps3d::cnn_matches(net, h->g.nx, h->g.ny, h->g.nz)) return fail(PS3D_E_ARG, "closure grid differs from the handle's");
+    h->cnn = net;
+    for (int k = 0; k < 6; ++k) h->cnn_in[k] = in_scale[k];
+    for (int k = 0; k < 3; ++k) h->cnn_out[k] = out_scale[k];
     return 0;
 }
 

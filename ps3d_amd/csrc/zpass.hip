@@ -795,6 +795,13 @@ __global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
             S[2][s][m] = T.fx[m] * Q[s][m ^ 1] - T.fy[m] * Pp[s][m ^ 2];
         }
 
+    if (a.extra[0]) {           // closure tendency
+#pragma unroll
+        for (int s = 0; s < ZPT; ++s)
+            if (T.has[s])
+                for (int m = 0; m < 4; ++m)
+                    for (int c = 0; c < 3; ++c) S[c][s][m] += a.extra[c][T.col[m] + T.z[s]];
+    }
     if (a.mode == ZT_STORE) {   // plain source: svorts
         for (int c = 0; c < 3; ++c) zstore(T, a.svorts[c], S[c]);
         return;

@@ -738,6 +738,16 @@ __global__ void __launch_bounds__(256) k_ztend_w(ZTendArgs a) {
         St[1].top = FP.top - c.fx * r1.top;
         St[2].top = c.fx * q1.top - c.fy * p2.top;
     }
+    if (a.extra[0]) {   // closure tendency
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            WF<NZ> ex;
+            wload(c, a.extra[cc], ex);
+#pragma unroll
+            for (int s = 0; s < S; ++s) St[cc].v[s] += ex.v[s];
+            St[cc].top += ex.top;
+        }
+    }
     if (a.mode == ZT_STORE) {
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) wstore(c, a.svorts[cc], St[cc]);
