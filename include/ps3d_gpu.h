@@ -238,8 +238,8 @@ int ps3d_gpu_ensemble_group(ps3d_ensemble *e, int g, ps3d_handle **h);
 int ps3d_gpu_ensemble_advance(ps3d_ensemble *e, int nsteps, int use_graph, double *ms);
 /* Times `steps` steps of each group under each available z launch shape and
  * keeps the fastest (state restored afterwards).  choice[g] (may be NULL)
- * receives the shape code: 100 = one wavefront per column, q < 100 = q quads per
- * block of the quad-per-thread kernels. */
+ * receives the shape code: 100..102 = one wavefront per column (register budget
+ * variant), q < 100 = q quads per block of the quad-per-thread kernels. */
 int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice);
 int ps3d_gpu_ensemble_set_choice(ps3d_ensemble *e, int g, int code);
 int ps3d_gpu_ensemble_destroy(ps3d_ensemble *e);

@@ -55,7 +55,7 @@ struct ZLaunch {
     size_t lds;
     int nz;
     int threads;   // TQ * QPB
-    int wave;      // 1: one wavefront per column (zwave.hip), 0: quad-per-thread (zpass.hip)
+    int wave;      // 1..3: one wavefront per column (zwave.hip, register variant), 0: quad-per-thread
     int nquads;
 };
 
@@ -128,8 +128,10 @@ hipError_t launch_zop(const ZLaunch &zl, const ZOpArgs &args, hipStream_t stream
 bool zwave_supported(int nz);
 size_t zwave_lds_bytes(int nz);
 hipError_t configure_zwave();
-hipError_t launch_zinv_w(int nz, int nquads, const ZInvArgs &args, hipStream_t stream);
-hipError_t launch_ztend_w(int nz, int nquads, const ZTendArgs &args, hipStream_t stream);
+// variant 1: tendency pass at three waves per SIMD; 2: register allocation left to
+// the compiler in both; 3: both at three waves per SIMD
+hipError_t launch_zinv_w(int nz, int variant, int nquads, const ZInvArgs &args, hipStream_t stream);
+hipError_t launch_ztend_w(int nz, int variant, int nquads, const ZTendArgs &args, hipStream_t stream);
 hipError_t launch_zop_w(int nz, int nquads, const ZOpArgs &args, hipStream_t stream);
 hipError_t launch_mean(const MeanArgs &args, int nz, hipStream_t stream);
 

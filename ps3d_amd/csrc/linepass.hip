@@ -228,6 +228,56 @@ __device__ inline void nn_export(const Tile &t, const LineArgs &a, const double2
         }
 }
 
+// Register prefetch of a tile for the 1024-thread kernels: the global loads of the
+// next field are issued before the transform of the current one.
+template <int N>
+struct Pf {
+    static constexpr int ZC = N > 0 ? zc_for(N) : 2;
+    static constexpr int PER = N > 0 ? (N * ZC + kLineThreads - 1) / kLineThreads : 1;
+    double pre[PER];
+};
+template <int N>
+__device__ inline void pf_load(Pf<N> &p, const Tile &t, const LineArgs &a, const double *__restrict__ src) {
+    constexpr int ZC = Pf<N>::ZC;
+#pragma unroll
+    for (int k = 0; k < Pf<N>::PER; ++k) {
+        const int e = threadIdx.x + k * kLineThreads, zz = e % ZC, j = e / ZC;
+        p.pre[k] = (j < t.n && zz < t.zce) ? src[t.base + (long)j * a.estride + zz] : 0.0;
+    }
+}
+template <int N>
+__device__ inline void pf_commit(const Pf<N> &p, const Tile &t, double2 *A) {
+    constexpr int ZC = Pf<N>::ZC;
+    double *d = reinterpret_cast<double *>(A);
+#pragma unroll
+    for (int k = 0; k < Pf<N>::PER; ++k) {
+        const int e = threadIdx.x + k * kLineThreads, zz = e % ZC, j = e / ZC;
+        if (j < t.n) d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = p.pre[k];
+    }
+}
+// field_phys on the prefetched tile (spectral input; diff: x/y derivative first),
+// starting the loads of `next` (nullptr: none).  N = 0 falls back to field_phys.
+template <int N>
+__device__ inline void phys_pf(Pf<N> &pf, const Tile &t, const LineArgs &a, double2 *A, double2 *B,
+                               double2 (&r)[kEpt], bool diff, const double *cur, const double *next) {
+    if constexpr (N == 0) {
+        field_phys<N>(t, a, cur, A, B, r, diff);
+    } else {
+        pf_commit(pf, t, A);
+        __syncthreads();
+        if (next) pf_load(pf, t, a, next);
+        double2 *S = A, *O = B;
+        if (diff) {
+            tile_diff(t, a, A, B);
+            S = B;
+            O = A;
+        }
+        double2 *R = tile_inverse<N, true>(t, a, S, O);
+        to_regs(t, R, r, 1.0 / (double)t.n);
+        __syncthreads();
+    }
+}
+
 // block-wide reductions (deterministic tree in LDS)
 __device__ inline double block_sum(double v, double *scratch) {
     const int tid = threadIdx.x;
@@ -375,22 +425,25 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
     }
 }
 
-// Modes that keep physical values in registers (products, adapt diagnostics).
-template <int N>
+// Modes that keep physical values in registers (products, adapt diagnostics); one
+// instantiation per mode M so that each gets its own register allocation.
+template <int N, int M>
 __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
     extern __shared__ double2 lsmem[];
     Tile t = make_tile(a);
     const size_t tb = (size_t)t.nl * t.LS;
-    double2 *A = lsmem, *B = lsmem + tb;
+    // the products pass transforms field pairs: A and B hold two tiles each
+    constexpr bool PAIRS = (M == LM_XMAIN && N > 0);
+    double2 *A = lsmem, *B = lsmem + (PAIRS ? 2 * tb : tb);
     if constexpr (N > 0) {
-        double2 *twl = B + tb;
+        double2 *twl = B + (PAIRS ? 2 * tb : tb);
         for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = a.tw[i];
         t.twl = twl;
     }
     double *scratch = reinterpret_cast<double *>(lsmem);
     const int nz = a.g.nz;
 
-    if (a.mode == LM_INV_Y1) {
+    if constexpr (M == LM_INV_Y1) {
         // u, v, w with their y derivatives; xi, eta, zeta; optional midpoints.
         // Parseval in x (this tile is one x slot): sum_x f^2 = sum_k w(k) f_k^2,
         // w = 1 at k = 0, nx/2 and 2 elsewhere.
@@ -398,16 +451,28 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         const double wk = (kx == 0 || kx == a.g.nx / 2) ? 1.0 : 2.0;
         double2 U[3][kEpt];
         double suu = 0.0, sww = 0.0, suw = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+        // every field is loaded through the register prefetch: the loads of the
+        // next source are in flight during the current transform
+        Pf<N> pf;
+        auto take = [&](const double *cur, const double *next) {
+            if constexpr (N > 0) {
+                pf_commit(pf, t, A);
+                __syncthreads();
+                if (next) pf_load(pf, t, a, next);
+            } else {
+                tile_load(t, a, cur, A);
+                __syncthreads();
+            }
+        };
+        if constexpr (N > 0) pf_load(pf, t, a, a.in[0]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            tile_load(t, a, a.in[c], A);
-            __syncthreads();
+            take(a.in[c], a.in[c]);
             tile_diff(t, a, A, B);            // y derivative in B
             double2 *R = tile_inverse<N, true>(t, a, B, A);
             tile_store(t, a, a.out[9 + c], R, 1.0 / (double)t.n);
             __syncthreads();
-            tile_load(t, a, a.in[c], A);
-            __syncthreads();
+            take(a.in[c], a.in[c + 1]);       // in[3] (xi) follows w
             R = tile_inverse<N, true>(t, a, A, B);
             tile_store(t, a, a.out[c], R, 1.0 / (double)t.n);
             to_regs(t, R, U[c], 1.0 / (double)t.n);
@@ -415,8 +480,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            tile_load(t, a, a.in[3 + c], A);
-            __syncthreads();
+            take(a.in[3 + c], (c < 2 || a.in[6]) ? a.in[4 + c] : nullptr);   // in[6]: midpoints
             double2 *R = tile_inverse<N, true>(t, a, A, B);
             tile_store(t, a, a.out[3 + c], R, 1.0 / (double)t.n);
             double2 W[kEpt];
@@ -442,8 +506,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         }
         if (a.in[6]) {
             for (int c = 0; c < 3; ++c) {
-                tile_load(t, a, a.in[6 + c], A);
-                __syncthreads();
+                take(a.in[6 + c], c < 2 ? a.in[7 + c] : nullptr);
                 double2 *R = tile_inverse<N, true>(t, a, A, B);
                 tile_store(t, a, a.out[6 + c], R, 1.0 / (double)t.n);
                 __syncthreads();
@@ -467,48 +530,101 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         return;
     }
 
-    if (a.mode == LM_XMAIN) {
-        // u x omega in flux form (inversion.f90:316-352) with omega + f_cor
-        double2 u[kEpt], v[kEpt], w[kEpt], r[kEpt], q[kEpt], p[kEpt];
-        field_phys<N>(t, a, a.in[0], A, B, u, false);
-        field_phys<N>(t, a, a.in[1], A, B, v, false);
-        field_phys<N>(t, a, a.in[2], A, B, w, false);
-        nn_export(t, a, u, 0);
-        nn_export(t, a, v, 1);
-        nn_export(t, a, w, 2);
-        double2 o[kEpt];
-        field_phys<N>(t, a, a.in[3], A, B, o, false);   // xi
-        nn_export(t, a, o, 3);
+    if constexpr (M == LM_XMAIN) {
+        // u x omega in flux form (inversion.f90:316-352) with omega + f_cor.  Fields
+        // go through the transforms in pairs (16 lines per FFT call: twice the
+        // butterflies per pass, half the barriers); the global loads of the next
+        // pair are in flight during the current pair's transform.
+        double2 u[kEpt], v[kEpt], w[kEpt], r[kEpt], q[kEpt], p[kEpt], o[kEpt], o2[kEpt];
+        if constexpr (N > 0) {
+            Tile t2 = t;
+            t2.nl = 2 * t.nl;
+            Pf<N> p0, p1;
+            auto phys2 = [&](int f, double2(&r0)[kEpt], double2(&r1)[kEpt]) {
+                pf_commit(p0, t, A);
+                pf_commit(p1, t, A + tb);
+                __syncthreads();
+                if (f + 2 < 6) {
+                    pf_load(p0, t, a, a.in[f + 2]);
+                    pf_load(p1, t, a, a.in[f + 3]);
+                }
+                if (a.phys_in) {
+                    to_regs(t, A, r0, 1.0);
+                    to_regs(t, A + tb, r1, 1.0);
+                } else {
+                    double2 *R = tile_inverse<N, true>(t2, a, A, B);
+                    to_regs(t, R, r0, 1.0 / (double)t.n);
+                    to_regs(t, R + tb, r1, 1.0 / (double)t.n);
+                }
+                __syncthreads();
+            };
+            pf_load(p0, t, a, a.in[0]);
+            pf_load(p1, t, a, a.in[1]);
+            phys2(0, u, v);
+            phys2(2, w, o);     // o = xi
+            nn_export(t, a, u, 0);
+            nn_export(t, a, v, 1);
+            nn_export(t, a, w, 2);
+            nn_export(t, a, o, 3);
 #pragma unroll
-        for (int k = 0; k < kEpt; ++k) {
-            const double xr = o[k].x + a.fcor[0], xi = o[k].y + a.fcor[0];
-            r[k] = make_double2(-v[k].x * xr, -v[k].y * xi);
-            q[k] = make_double2(w[k].x * xr, w[k].y * xi);
+            for (int k = 0; k < kEpt; ++k) {
+                const double xr = o[k].x + a.fcor[0], xi = o[k].y + a.fcor[0];
+                r[k] = make_double2(-v[k].x * xr, -v[k].y * xi);
+                q[k] = make_double2(w[k].x * xr, w[k].y * xi);
+            }
+            phys2(4, o, o2);    // o = eta, o2 = zeta
+        } else {
+            field_phys<N>(t, a, a.in[0], A, B, u, false);
+            field_phys<N>(t, a, a.in[1], A, B, v, false);
+            field_phys<N>(t, a, a.in[2], A, B, w, false);
+            field_phys<N>(t, a, a.in[3], A, B, o, false);
+            nn_export(t, a, u, 0);
+            nn_export(t, a, v, 1);
+            nn_export(t, a, w, 2);
+            nn_export(t, a, o, 3);
+#pragma unroll
+            for (int k = 0; k < kEpt; ++k) {
+                const double xr = o[k].x + a.fcor[0], xi = o[k].y + a.fcor[0];
+                r[k] = make_double2(-v[k].x * xr, -v[k].y * xi);
+                q[k] = make_double2(w[k].x * xr, w[k].y * xi);
+            }
+            field_phys<N>(t, a, a.in[4], A, B, o, false);
+            field_phys<N>(t, a, a.in[5], A, B, o2, false);
         }
-        field_phys<N>(t, a, a.in[4], A, B, o, false);   // eta
         nn_export(t, a, o, 4);
+        nn_export(t, a, o2, 5);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
             const double er = o[k].x + a.fcor[1], ei = o[k].y + a.fcor[1];
             r[k] = make_double2(u[k].x * er + r[k].x, u[k].y * ei + r[k].y);
             p[k] = make_double2(-w[k].x * er, -w[k].y * ei);
         }
-        field_phys<N>(t, a, a.in[5], A, B, o, false);   // zeta
-        nn_export(t, a, o, 5);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
-            const double zr = o[k].x + a.fcor[2], zi = o[k].y + a.fcor[2];
+            const double zr = o2[k].x + a.fcor[2], zi = o2[k].y + a.fcor[2];
             q[k] = make_double2(q[k].x - u[k].x * zr, q[k].y - u[k].y * zi);
             p[k] = make_double2(v[k].x * zr + p[k].x, v[k].y * zi + p[k].y);
         }
         // r = u eta - v xi ; q = w xi - u zeta ; p = v zeta - w eta  -> forward in x
-        fwd_store<N>(t, a, A, B, r, a.out[0]);
-        fwd_store<N>(t, a, A, B, q, a.out[1]);
+        if constexpr (N > 0) {
+            Tile t2 = t;
+            t2.nl = 2 * t.nl;
+            from_regs(t, A, r);
+            from_regs(t, A + tb, q);
+            __syncthreads();
+            double2 *R = tile_forward<N, true>(t2, a, A, B);
+            tile_store(t, a, a.out[0], R, 1.0);
+            tile_store(t, a, a.out[1], R + tb, 1.0);
+            __syncthreads();
+        } else {
+            fwd_store<N>(t, a, A, B, r, a.out[0]);
+            fwd_store<N>(t, a, A, B, q, a.out[1]);
+        }
         fwd_store<N>(t, a, A, B, p, a.out[2]);
         return;
     }
 
-    if (a.mode == LM_XADAPT) {
+    if constexpr (M == LM_XADAPT) {
         // adapt (advance.f90:171-289): max |omega|, signed max of u, v, w, largest
         // |eigenvalue| of the symmetrised strain (Jacobi per point), lid maxima, and
         // the sums of the characteristic vorticity.
@@ -516,29 +632,41 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         double umax = -INFINITY, vmax = -INFINITY, wmax = -INFINITY, w2max = 0.0;
         double gg = 2.220446049250313e-16, usgg = 0.0, lsgg = 0.0, l1 = 0.0, l2 = 0.0;
         double2 ux[kEpt], uy[kEpt], vy[kEpt], wx[kEpt], wy[kEpt], xi[kEpt], et[kEpt], ze[kEpt];
-        field_phys<N>(t, a, a.in[0], A, B, ux, false);   // u (for umax), then du/dx
+        // field order: u v w u_x w_x u_y v_y w_y xi eta zeta, midpoint xi eta zeta;
+        // spectral inputs go through the register prefetch of the next field
+        Pf<N> pf;
+        auto F = [&](double2(&r_)[kEpt], bool diff, int k, int kn) {
+            const double *next = kn >= 0 ? a.in[kn] : nullptr;
+            if (a.phys_in)
+                field_phys<N>(t, a, a.in[k], A, B, r_, diff);
+            else
+                phys_pf<N>(pf, t, a, A, B, r_, diff, a.in[k], next);
+        };
+        if constexpr (N > 0)
+            if (!a.phys_in) pf_load(pf, t, a, a.in[0]);
+        F(ux, false, 0, 1);   // u (for umax), then du/dx
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h)
                 if (reg_valid(t, k, h, nz)) umax = fmax(umax, h ? ux[k].y : ux[k].x);
-        field_phys<N>(t, a, a.in[1], A, B, vy, false);
+        F(vy, false, 1, 2);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h)
                 if (reg_valid(t, k, h, nz)) vmax = fmax(vmax, h ? vy[k].y : vy[k].x);
-        field_phys<N>(t, a, a.in[2], A, B, wx, false);
+        F(wx, false, 2, 0);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h)
                 if (reg_valid(t, k, h, nz)) wmax = fmax(wmax, h ? wx[k].y : wx[k].x);
-        field_phys<N>(t, a, a.in[0], A, B, ux, true);    // du/dx
-        field_phys<N>(t, a, a.in[2], A, B, wx, true);    // dw/dx
-        field_phys<N>(t, a, a.in[6], A, B, uy, false);   // du/dy
-        field_phys<N>(t, a, a.in[7], A, B, vy, false);   // dv/dy
-        field_phys<N>(t, a, a.in[8], A, B, wy, false);   // dw/dy
-        field_phys<N>(t, a, a.in[3], A, B, xi, false);
-        field_phys<N>(t, a, a.in[4], A, B, et, false);
-        field_phys<N>(t, a, a.in[5], A, B, ze, false);
+        F(ux, true, 0, 2);     // du/dx
+        F(wx, true, 2, 6);     // dw/dx
+        F(uy, false, 6, 7);    // du/dy
+        F(vy, false, 7, 8);    // dv/dy
+        F(wy, false, 8, 3);    // dw/dy
+        F(xi, false, 3, 4);
+        F(et, false, 4, 5);
+        F(ze, false, 5, 9);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h) {
@@ -557,9 +685,9 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
             }
         // characteristic vorticity <|w|^2>/<|w|> over cells with |w| > w_rms
         // (field_diagnostics.f90:501-545), from the z-midpoint vorticity
-        field_phys<N>(t, a, a.in[9], A, B, xi, false);
-        field_phys<N>(t, a, a.in[10], A, B, et, false);
-        field_phys<N>(t, a, a.in[11], A, B, ze, false);
+        F(xi, false, 9, 10);
+        F(et, false, 10, 11);
+        F(ze, false, 11, -1);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h) {
@@ -611,7 +739,9 @@ hipError_t configure_linepass() {
     hipError_t e = hipSuccess;
 #define CFG(N_)                                                                                              \
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_simple<N_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_INV_Y1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XMAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XADAPT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     CFG(0)
     PS3D_LINE_SIZES(CFG)
 #undef CFG
@@ -622,11 +752,16 @@ template <int N>
 static void launch_line_t(const LineArgs &a, hipStream_t stream) {
     const int blocks = a.nother * a.nzt;
     const int nb = a.bt.nb > 1 ? a.bt.nb : 1;
-    const size_t lds = line_lds_bytes(a.n, a.ZC);
+    size_t lds = line_lds_bytes(a.n, a.ZC);
+    if (a.mode == LM_XMAIN && N > 0) lds += (size_t)2 * (a.ZC / 2) * (a.n + 1) * sizeof(double2);   // field pairs
     if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF)
         hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks, nb), dim3(256), lds, stream, a);
+    else if (a.mode == LM_INV_Y1)
+        hipLaunchKernelGGL((k_line_reg<N, LM_INV_Y1>), dim3(blocks, nb), dim3(kLineThreads), lds, stream, a);
+    else if (a.mode == LM_XMAIN)
+        hipLaunchKernelGGL((k_line_reg<N, LM_XMAIN>), dim3(blocks, nb), dim3(kLineThreads), lds, stream, a);
     else
-        hipLaunchKernelGGL(k_line_reg<N>, dim3(blocks, nb), dim3(kLineThreads), lds, stream, a);
+        hipLaunchKernelGGL((k_line_reg<N, LM_XADAPT>), dim3(blocks, nb), dim3(kLineThreads), lds, stream, a);
 }
 
 hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {

@@ -309,7 +309,10 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     {
         const char *ev = std::getenv("PS3D_ZWAVE");
         const bool off = ev && ev[0] == '0';
-        h->zl.wave = (ps3d::zwave_supported(nz) && !off) ? 1 : 0;
+        // register variant: at nz = 512 the three-waves-per-SIMD build spills heavily
+        int var = nz == 512 ? 2 : 1;
+        if (ev && ev[0] >= '1' && ev[0] <= '3') var = ev[0] - '0';
+        h->zl.wave = (ps3d::zwave_supported(nz) && !off) ? var : 0;
     }
 
     h->ly = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs);
@@ -1448,7 +1451,11 @@ int ps3d_gpu_ensemble_advance(ps3d_ensemble *e, int nsteps, int use_graph, doubl
 // available, else quad-per-thread with QPB quads per block).  Code = wave * 100 + QPB.
 static std::vector<int> z_candidates(const ps3d_handle *h) {
     std::vector<int> c;
-    if (ps3d::zwave_supported(h->g.nz)) c.push_back(100);
+    if (ps3d::zwave_supported(h->g.nz)) {
+        c.push_back(100);
+        c.push_back(101);
+        c.push_back(102);
+    }
     for (int q = 1; q <= 8; q *= 2) {
         if (h->TQ * q > 512) break;
         if (zpass_lds_bytes(q, h->g.nz) > 160 * 1024) break;
@@ -1458,7 +1465,7 @@ static std::vector<int> z_candidates(const ps3d_handle *h) {
 }
 
 static void apply_z_choice(ps3d_handle *h, int code) {
-    h->zl.wave = code >= 100 ? 1 : 0;
+    h->zl.wave = code >= 100 ? 1 + (code - 100) : 0;
     if (!h->zl.wave) {
         h->QPB = code;
         h->zl.threads = h->TQ * h->QPB;
@@ -1467,7 +1474,7 @@ static void apply_z_choice(ps3d_handle *h, int code) {
     }
 }
 
-static int current_z_choice(const ps3d_handle *h) { return h->zl.wave ? 100 : h->QPB; }
+static int current_z_choice(const ps3d_handle *h) { return h->zl.wave ? 99 + h->zl.wave : h->QPB; }
 
 // Times `steps` fused steps of each group under every candidate shape and keeps the
 // fastest; the state (svor, device scalars, rolling-mean history) is restored.

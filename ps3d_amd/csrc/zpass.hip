@@ -1073,12 +1073,12 @@ hipError_t configure_zpass() {
 }
 
 hipError_t launch_zinv(const ZLaunch &zl, const ZInvArgs &args, hipStream_t stream) {
-    if (zl.wave) return launch_zinv_w(zl.nz, zl.nquads, args, stream);
+    if (zl.wave) return launch_zinv_w(zl.nz, zl.wave, zl.nquads, args, stream);
     ZDISPATCH_FFT(k_zinv, args, zl);
     return hipGetLastError();
 }
 hipError_t launch_ztend(const ZLaunch &zl, const ZTendArgs &args, hipStream_t stream) {
-    if (zl.wave) return launch_ztend_w(zl.nz, zl.nquads, args, stream);
+    if (zl.wave) return launch_ztend_w(zl.nz, zl.wave, zl.nquads, args, stream);
     ZDISPATCH_FFT(k_ztend, args, zl);
     return hipGetLastError();
 }

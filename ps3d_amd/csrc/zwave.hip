@@ -478,8 +478,11 @@ __device__ inline void wzero(WF<NZ> &F) {
 // vor2vel, vertical part (src/inversion/inversion.f90:23-226) for one quad, one
 // wave per member.  Same inputs and outputs as the quad-per-thread k_zinv.
 // ---------------------------------------------------------------------------
-template <int NZ>
-__global__ void __launch_bounds__(256) k_zinv_w(ZInvArgs a) {
+// OCC > 1 asks the compiler for OCC waves per SIMD (fewer registers, possibly
+// spills); OCC = 1 leaves it free.  Both variants are built; the launch shape
+// (and the ensemble tuner) choose.
+template <int NZ, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_zinv_w(ZInvArgs a) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     extern __shared__ double2 wsm[];
     const Grid g = a.g;
@@ -682,8 +685,8 @@ __global__ void __launch_bounds__(256) k_zinv_w(ZInvArgs a) {
 // vorticity_tendency, vertical part (inversion.f90:298-371), fused with one RK4
 // substep or cn2 update; one wave per member.  Same contract as k_ztend.
 // ---------------------------------------------------------------------------
-template <int NZ>
-__global__ void __launch_bounds__(256) k_ztend_w(ZTendArgs a) {
+template <int NZ, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_ztend_w(ZTendArgs a) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     extern __shared__ double2 wsm[];
     const Grid g = a.g;
@@ -831,10 +834,16 @@ hipError_t configure_zwave() {
     hipError_t e = hipSuccess;
 #define SETW(NZ_)                                                                                            \
     if (e == hipSuccess)                                                                                     \
-        e = hipFuncSetAttribute((const void *)(k_zinv_w<NZ_>), hipFuncAttributeMaxDynamicSharedMemorySize,   \
+        e = hipFuncSetAttribute((const void *)(k_zinv_w<NZ_, 1>), hipFuncAttributeMaxDynamicSharedMemorySize, \
                                 (int)zwave_lds_bytes(NZ_));                                                  \
     if (e == hipSuccess)                                                                                     \
-        e = hipFuncSetAttribute((const void *)(k_ztend_w<NZ_>), hipFuncAttributeMaxDynamicSharedMemorySize,  \
+        e = hipFuncSetAttribute((const void *)(k_ztend_w<NZ_, 1>), hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)zwave_lds_bytes(NZ_));                                                  \
+    if (e == hipSuccess)                                                                                     \
+        e = hipFuncSetAttribute((const void *)(k_zinv_w<NZ_, 3>), hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)zwave_lds_bytes(NZ_));                                                  \
+    if (e == hipSuccess)                                                                                     \
+        e = hipFuncSetAttribute((const void *)(k_ztend_w<NZ_, 3>), hipFuncAttributeMaxDynamicSharedMemorySize, \
                                 (int)zwave_lds_bytes(NZ_));                                                  \
     if (e == hipSuccess)                                                                                     \
         e = hipFuncSetAttribute((const void *)(k_zop_w<NZ_>), hipFuncAttributeMaxDynamicSharedMemorySize,    \
@@ -852,12 +861,25 @@ hipError_t configure_zwave() {
         default: return hipErrorInvalidValue;                                                           \
     }
 
-hipError_t launch_zinv_w(int nz, int nquads, const ZInvArgs &args, hipStream_t stream) {
-    ZW_DISPATCH(k_zinv_w, nquads, (args.bt.nb > 1 ? args.bt.nb : 1), args);
+// variant 1: vor2vel pass free, tendency pass at three waves per SIMD (measured best
+// at nz = 256); 2: both free; 3: both at three waves per SIMD
+#define ZW_DISPATCH2(KERNEL, NQ, NB, ARGS, OCC3)                                                        \
+    switch (nz * 10 + ((OCC3) ? 3 : 1)) {                                                          \
+        case 1281: hipLaunchKernelGGL((KERNEL<128, 1>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(128), stream, ARGS); break; \
+        case 2561: hipLaunchKernelGGL((KERNEL<256, 1>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(256), stream, ARGS); break; \
+        case 5121: hipLaunchKernelGGL((KERNEL<512, 1>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(512), stream, ARGS); break; \
+        case 1283: hipLaunchKernelGGL((KERNEL<128, 3>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(128), stream, ARGS); break; \
+        case 2563: hipLaunchKernelGGL((KERNEL<256, 3>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(256), stream, ARGS); break; \
+        case 5123: hipLaunchKernelGGL((KERNEL<512, 3>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(512), stream, ARGS); break; \
+        default: return hipErrorInvalidValue;                                                           \
+    }
+
+hipError_t launch_zinv_w(int nz, int variant, int nquads, const ZInvArgs &args, hipStream_t stream) {
+    ZW_DISPATCH2(k_zinv_w, nquads, (args.bt.nb > 1 ? args.bt.nb : 1), args, variant == 3);
     return hipGetLastError();
 }
-hipError_t launch_ztend_w(int nz, int nquads, const ZTendArgs &args, hipStream_t stream) {
-    ZW_DISPATCH(k_ztend_w, nquads, (args.bt.nb > 1 ? args.bt.nb : 1), args);
+hipError_t launch_ztend_w(int nz, int variant, int nquads, const ZTendArgs &args, hipStream_t stream) {
+    ZW_DISPATCH2(k_ztend_w, nquads, (args.bt.nb > 1 ? args.bt.nb : 1), args, variant == 1 || variant == 3);
     return hipGetLastError();
 }
 hipError_t launch_zop_w(int nz, int nquads, const ZOpArgs &args, hipStream_t stream) {

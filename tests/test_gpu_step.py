@@ -227,14 +227,16 @@ def test_long_column_trajectory(oracle_mod, G, shape, filtering):
     o.close()
 
 
-def test_wave_and_quad_z_kernels_agree(G, monkeypatch):
-    """The wave-per-column and quad-per-thread z kernels give the same step."""
+@pytest.mark.parametrize("variant", ["1", "2", "3"])
+def test_wave_and_quad_z_kernels_agree(G, monkeypatch, variant):
+    """The wave-per-column (both register budgets) and quad-per-thread z kernels
+    give the same step."""
     p = G.make_params(**box(32), **config1())
-    p.nz = 128
-    vor = ics.beltrami_multimode(32, 32, 128, seed=5)
+    p.nz = 256
+    vor = ics.beltrami_multimode(32, 32, 256, seed=5)
     monkeypatch.setenv("PS3D_ZWAVE", "0")
     a = G.Ps3dGpu(p)
-    monkeypatch.setenv("PS3D_ZWAVE", "1")
+    monkeypatch.setenv("PS3D_ZWAVE", variant)
     b = G.Ps3dGpu(p)
     a.set_vorticity(vor)
     b.set_vorticity(vor)
