@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
     Tile t = make_tile(a);
     const size_t tb = (size_t)t.nl * t.LS;
     // the products pass transforms field pairs: A and B hold two tiles each
-    constexpr bool PAIRS = (M == LM_XMAIN && N > 0);
+    constexpr bool PAIRS = ((M == LM_XMAIN || M == LM_XADAPT) && N > 0);
     double2 *A = lsmem, *B = lsmem + (PAIRS ? 2 * tb : tb);
     if constexpr (N > 0) {
         double2 *twl = B + (PAIRS ? 2 * tb : tb);
@@ -632,41 +632,58 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         double umax = -INFINITY, vmax = -INFINITY, wmax = -INFINITY, w2max = 0.0;
         double gg = 2.220446049250313e-16, usgg = 0.0, lsgg = 0.0, l1 = 0.0, l2 = 0.0;
         double2 ux[kEpt], uy[kEpt], vy[kEpt], wx[kEpt], wy[kEpt], xi[kEpt], et[kEpt], ze[kEpt];
-        // field order: u v w u_x w_x u_y v_y w_y xi eta zeta, midpoint xi eta zeta;
-        // spectral inputs go through the register prefetch of the next field
-        Pf<N> pf;
-        auto F = [&](double2(&r_)[kEpt], bool diff, int k, int kn) {
-            const double *next = kn >= 0 ? a.in[kn] : nullptr;
-            if (a.phys_in)
-                field_phys<N>(t, a, a.in[k], A, B, r_, diff);
-            else
-                phys_pf<N>(pf, t, a, A, B, r_, diff, a.in[k], next);
+        // Fields in pairs (16 lines per FFT call): (u, v) (w, xi) (u_x, w_x) (u_y, v_y)
+        // (w_y, eta) (zeta) then the midpoint vorticity (mxi, meta) (mzeta); spectral
+        // inputs prefetch the next pair during the current transform.
+        Pf<N> p0, p1;
+        Tile t2 = t;
+        t2.nl = 2 * t.nl;
+        auto get2 = [&](int fa, int fb, bool diff, double2(&ra)[kEpt], double2(&rb)[kEpt], int na, int nb) {
+            if (N == 0 || a.phys_in) {
+                field_phys<N>(t, a, a.in[fa], A, B, ra, diff);
+                if (fb >= 0) field_phys<N>(t, a, a.in[fb], A, B, rb, diff);
+                return;
+            }
+            pf_commit(p0, t, A);
+            if (fb >= 0) pf_commit(p1, t, A + tb);
+            __syncthreads();
+            if (na >= 0) pf_load(p0, t, a, a.in[na]);
+            if (nb >= 0) pf_load(p1, t, a, a.in[nb]);
+            const Tile &tt = fb >= 0 ? t2 : t;
+            double2 *S = A, *O = B;
+            if (diff) {
+                tile_diff(tt, a, A, B);
+                S = B;
+                O = A;
+            }
+            double2 *R = tile_inverse<N, true>(tt, a, S, O);
+            to_regs(t, R, ra, 1.0 / (double)t.n);
+            if (fb >= 0) to_regs(t, R + tb, rb, 1.0 / (double)t.n);
+            __syncthreads();
         };
-        if constexpr (N > 0)
-            if (!a.phys_in) pf_load(pf, t, a, a.in[0]);
-        F(ux, false, 0, 1);   // u (for umax), then du/dx
+        if constexpr (N > 0) {
+            if (!a.phys_in) {
+                pf_load(p0, t, a, a.in[0]);
+                pf_load(p1, t, a, a.in[1]);
+            }
+        }
+        get2(0, 1, false, ux, vy, 2, 3);       // u, v (maxima)
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h)
-                if (reg_valid(t, k, h, nz)) umax = fmax(umax, h ? ux[k].y : ux[k].x);
-        F(vy, false, 1, 2);
-#pragma unroll
-        for (int k = 0; k < kEpt; ++k)
-            for (int h = 0; h < 2; ++h)
-                if (reg_valid(t, k, h, nz)) vmax = fmax(vmax, h ? vy[k].y : vy[k].x);
-        F(wx, false, 2, 0);
+                if (reg_valid(t, k, h, nz)) {
+                    umax = fmax(umax, h ? ux[k].y : ux[k].x);
+                    vmax = fmax(vmax, h ? vy[k].y : vy[k].x);
+                }
+        get2(2, 3, false, wx, xi, 0, 2);       // w (maximum), xi
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h)
                 if (reg_valid(t, k, h, nz)) wmax = fmax(wmax, h ? wx[k].y : wx[k].x);
-        F(ux, true, 0, 2);     // du/dx
-        F(wx, true, 2, 6);     // dw/dx
-        F(uy, false, 6, 7);    // du/dy
-        F(vy, false, 7, 8);    // dv/dy
-        F(wy, false, 8, 3);    // dw/dy
-        F(xi, false, 3, 4);
-        F(et, false, 4, 5);
-        F(ze, false, 5, 9);
+        get2(0, 2, true, ux, wx, 6, 7);        // du/dx, dw/dx
+        get2(6, 7, false, uy, vy, 8, 4);       // du/dy, dv/dy
+        get2(8, 4, false, wy, et, 5, -1);      // dw/dy, eta
+        get2(5, -1, false, ze, ze, 9, 10);     // zeta
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h) {
@@ -685,9 +702,8 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
             }
         // characteristic vorticity <|w|^2>/<|w|> over cells with |w| > w_rms
         // (field_diagnostics.f90:501-545), from the z-midpoint vorticity
-        F(xi, false, 9, 10);
-        F(et, false, 10, 11);
-        F(ze, false, 11, -1);
+        get2(9, 10, false, xi, et, 11, -1);
+        get2(11, -1, false, ze, ze, -1, -1);
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h) {
@@ -753,7 +769,8 @@ static void launch_line_t(const LineArgs &a, hipStream_t stream) {
     const int blocks = a.nother * a.nzt;
     const int nb = a.bt.nb > 1 ? a.bt.nb : 1;
     size_t lds = line_lds_bytes(a.n, a.ZC);
-    if (a.mode == LM_XMAIN && N > 0) lds += (size_t)2 * (a.ZC / 2) * (a.n + 1) * sizeof(double2);   // field pairs
+    if ((a.mode == LM_XMAIN || a.mode == LM_XADAPT) && N > 0)
+        lds += (size_t)2 * (a.ZC / 2) * (a.n + 1) * sizeof(double2);   // field pairs
     if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF)
         hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks, nb), dim3(256), lds, stream, a);
     else if (a.mode == LM_INV_Y1)
