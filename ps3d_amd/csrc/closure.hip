@@ -8,7 +8,8 @@
 // Activations and weights feeding the matrix cores are bf16; accumulation, biases,
 // pre-activations, gradients and the Adam state are fp32.
 //
-// Convolution = implicit GEMM on the MFMA units: a wavefront owns 64 points x all
+// Convolution = implicit GEMM on the MFMA units: a block owns a brick of 4 x 4
+// columns x 16 levels (input halo staged in LDS), a wavefront 4 columns x all
 // output channels; per tap and 16/32-channel slice it issues
 // v_mfma_f32_16x16x16_bf16 (16 input channels) or v_mfma_f32_16x16x32_bf16 (32).
 // Operand lanes: A row = lane%16 (point), k = (K/4)(lane/16)+v (channel);
@@ -66,86 +67,88 @@ struct ConvArgs {
     long npts;
 };
 
+// Brick of 4 x 4 (x, y) columns x 16 z levels per block (256 points, one 16-row
+// M tile per column); its input halo (6 x 6 x 18 points, all channels) is staged in
+// LDS once and read by all 27 taps.
+constexpr int BX = 4, BY = 4, BZ = 16, HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
+
+__host__ __device__ inline int conv_blocks(int nx, int ny, int nz1) {
+    return ((nx + BX - 1) / BX) * ((ny + BY - 1) / BY) * ((nz1 + BZ - 1) / BZ);
+}
+
 template <int CIN, int COUT, int MODE>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     constexpr int KK = (CIN % 32 == 0) ? 32 : 16;
-    constexpr int NKC = CIN / KK;
     constexpr int NT = COUT / 16;
-    constexpr int MT = 4;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long p0 = ((long)blockIdx.x * 4 + wave) * 64;
-    const int row = lane & 15, kq = lane >> 4;
-    int px[MT], py[MT], pz[MT];
-    bool pv[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const long p = p0 + mt * 16 + row;
-        pv[mt] = p < a.npts;
-        const long pp = pv[mt] ? p : 0;
-        pz[mt] = (int)(pp % a.nz1);
-        const long r = pp / a.nz1;
-        py[mt] = (int)(r % a.ny);
-        px[mt] = (int)(r / a.ny);
+    constexpr int MT = 4;                      // columns (M tiles) per wave
+    constexpr int CH = CIN * 2 / 16;           // 16-byte chunks per halo row
+    __shared__ uint4 halo4[HX * HY * HZ * CH];
+    const uint16_t *hal = reinterpret_cast<const uint16_t *>(halo4);
+    const int nbx = (a.nx + BX - 1) / BX, nby = (a.ny + BY - 1) / BY;
+    int b = blockIdx.x;
+    const int bz = b / (nbx * nby);
+    b -= bz * nbx * nby;
+    const int x0 = (b / nby) * BX, y0 = (b % nby) * BY, z0 = bz * BZ;
+    for (int e = threadIdx.x; e < HX * HY * HZ * CH; e += 256) {
+        const int ch = e % CH, r = e / CH;
+        const int hz = r % HZ, hy = (r / HZ) % HY, hx = r / (HZ * HY);
+        const int z = z0 + hz - 1;
+        int x = (x0 + hx - 1) % a.nx, y = (y0 + hy - 1) % a.ny;
+        x = x < 0 ? x + a.nx : x;
+        y = y < 0 ? y + a.ny : y;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (z >= 0 && z < a.nz1)
+            v = *reinterpret_cast<const uint4 *>(a.in + (((size_t)x * a.ny + y) * a.nz1 + z) * CIN + ch * 8);
+        halo4[e] = v;
     }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 15, kq = lane >> 4;
     f4v acc[MT][NT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
-
+#pragma unroll 3
     for (int t = 0; t < 27; ++t) {
         const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
-        long nb[MT];
+        const int c0 = kq * (KK / 4);
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
-            int x = px[mt] + dx, y = py[mt] + dy;
-            const int z = pz[mt] + dz;
-            x = x < 0 ? x + a.nx : (x >= a.nx ? x - a.nx : x);
-            y = y < 0 ? y + a.ny : (y >= a.ny ? y - a.ny : y);
-            nb[mt] = (pv[mt] && z >= 0 && z < a.nz1) ? (((long)x * a.ny + y) * a.nz1 + z) : -1;
-        }
-#pragma unroll
-        for (int kc = 0; kc < NKC; ++kc) {
-            const int c0 = kc * KK + kq * (KK / 4);
+            const int c = wave * MT + mt;                 // brick column
+            const int hr = (((c / BY) + 1 + dx) * HY + ((c % BY) + 1 + dy)) * HZ + (row + 1 + dz);
             if constexpr (KK == 16) {
-                s4v bf[NT];
+                const s4v af = *reinterpret_cast<const s4v *>(hal + (size_t)hr * CIN + c0);
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    bf[nt] = *reinterpret_cast<const s4v *>(a.w + ((size_t)(t * COUT + nt * 16 + row) * CIN + c0));
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                    s4v af = s4v{0, 0, 0, 0};
-                    if (nb[mt] >= 0) af = *reinterpret_cast<const s4v *>(a.in + (size_t)nb[mt] * CIN + c0);
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf[nt], acc[mt][nt], 0, 0, 0);
+                for (int nt = 0; nt < NT; ++nt) {
+                    const s4v bf =
+                        *reinterpret_cast<const s4v *>(a.w + ((size_t)(t * COUT + nt * 16 + row) * CIN + c0));
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bf, acc[mt][nt], 0, 0, 0);
                 }
             } else {
-                b8v bf[NT];
+                const b8v af = *reinterpret_cast<const b8v *>(hal + (size_t)hr * CIN + c0);
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    bf[nt] = *reinterpret_cast<const b8v *>(a.w + ((size_t)(t * COUT + nt * 16 + row) * CIN + c0));
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                    uint4 raw = make_uint4(0u, 0u, 0u, 0u);
-                    if (nb[mt] >= 0) raw = *reinterpret_cast<const uint4 *>(a.in + (size_t)nb[mt] * CIN + c0);
-                    const b8v af = __builtin_bit_cast(b8v, raw);
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[nt], acc[mt][nt], 0, 0, 0);
+                for (int nt = 0; nt < NT; ++nt) {
+                    const b8v bf =
+                        *reinterpret_cast<const b8v *>(a.w + ((size_t)(t * COUT + nt * 16 + row) * CIN + c0));
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc[mt][nt], 0, 0, 0);
                 }
             }
         }
     }
-    // D: row 4*(lane/16)+v (point), column lane%16 (channel)
+    // D: row 4*(lane/16)+v (z in the brick), column lane%16 (channel)
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MT; ++mt) {
+        const int c = wave * MT + mt;
+        const int x = x0 + c / BY, y = y0 + c % BY;
+        if (x >= a.nx || y >= a.ny) continue;
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const long p = p0 + mt * 16 + 4 * kq + v;
-                if (p >= a.npts) continue;
+                const int z = z0 + 4 * kq + v;
+                if (z >= a.nz1) continue;
+                const long p = ((long)x * a.ny + y) * a.nz1 + z;
                 const int co = nt * 16 + row;
                 const size_t o = (size_t)p * COUT + co;
                 float val = acc[mt][nt][v];
@@ -163,6 +166,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
                     a.outf[o] = val;
                 }
             }
+    }
 }
 
 // Weight (and bias) gradient partials of one layer over a chunk of points and one
@@ -397,7 +401,7 @@ void conv(const ps3d_cnn *c, const uint16_t *in, const uint16_t *w, const float 
     a.ny = c->ny;
     a.nz1 = c->nz1;
     a.npts = c->npts;
-    hipLaunchKernelGGL((k_conv<CIN, COUT, MODE>), dim3(blocks_for(c->npts, 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_conv<CIN, COUT, MODE>), dim3(conv_blocks(c->nx, c->ny, c->nz1)), dim3(256), 0, s, a);
 }
 
 template <int CIN, int COUT>
