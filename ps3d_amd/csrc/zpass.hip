@@ -93,19 +93,30 @@ __device__ inline ZT<ZPT> make_zt(const Grid &g, const ZTabs &tb, int TQ, int QP
 // LDS helpers
 // ---------------------------------------------------------------------------
 struct ZLds {
-    double2 *A, *B;    // FFT ping-pong: up to QPB*4 lines of N complex each
+    double2 *A, *B;    // FFT ping-pong: QPB*lines lines of N complex each
     double *lid;       // QPB * 3 fields * 4 members * 2 (z=0, z=nz)
-    double *colbuf;    // QPB * 4 * (nz+1)   (aliases A)
+    double *colbuf;    // QPB * 12 * (nz+1)   (aliases A, B)
+    int lines;         // 4: two fields per FFT call; 2 (long columns): one
 };
+
+// FFT lines per quad that fit the 160 KB of LDS: 4 (two transforms per call) when
+// possible, else 2.
+__host__ __device__ inline int zpass_lines(int QPB, int nz) {
+    const size_t N = 2 * (size_t)nz;
+    const size_t fft4 = 2 * (size_t)QPB * 4 * N * sizeof(double2);
+    const size_t col = (size_t)QPB * 12 * (nz + 1) * sizeof(double);
+    const size_t need4 = (fft4 > col ? fft4 : col) + (size_t)QPB * 24 * sizeof(double);
+    return need4 <= 160 * 1024 ? 4 : 2;
+}
 
 __device__ inline ZLds zlds(int QPB, int N, int nz) {
     extern __shared__ double2 zsmem[];
     ZLds L;
+    L.lines = zpass_lines(QPB, nz);
     L.A = zsmem;
-    L.B = zsmem + (size_t)QPB * 4 * N;
-    L.lid = reinterpret_cast<double *>(L.B + (size_t)QPB * 4 * N);
+    L.B = zsmem + (size_t)QPB * L.lines * N;
+    L.lid = reinterpret_cast<double *>(L.B + (size_t)QPB * L.lines * N);
     L.colbuf = reinterpret_cast<double *>(L.A);
-    (void)nz;
     return L;
 }
 
@@ -330,9 +341,27 @@ __device__ inline void ext_get(const ZT<ZPT> &T, const double2 *r0, const double
 }
 
 // Two z transforms of the quad in one batched FFT call (4 lines per quad).
+template <int ZPT, int NZ, int K>
+__device__ inline void zxform1(const ZT<ZPT> &T, const ZLds &L, double (&F)[ZPT][4], const FftPlan &P,
+                               const double2 *tw, double scale) {
+    const int N = T.N;
+    double2 *b = L.A + (size_t)(T.qi * 2) * N;
+    ext_put<ZPT, K>(T, b, b + N, F);
+    __syncthreads();
+    double2 *R = zfft<NZ>(L, T.QPB * 2, N, P, tw);
+    const double2 *r = R + (size_t)(T.qi * 2) * N;
+    ext_get<ZPT, K>(T, r, r + N, F, scale);
+    __syncthreads();
+}
+
 template <int ZPT, int NZ, int K0, int K1>
 __device__ inline void zxform2(const ZT<ZPT> &T, const ZLds &L, double (&F0)[ZPT][4], double (&F1)[ZPT][4],
                                const FftPlan &P, const double2 *tw, double scale) {
+    if (L.lines != 4) {   // long columns: one transform per call
+        zxform1<ZPT, NZ, K0>(T, L, F0, P, tw, scale);
+        zxform1<ZPT, NZ, K1>(T, L, F1, P, tw, scale);
+        return;
+    }
     const int N = T.N;
     double2 *b = L.A + (size_t)(T.qi * 4) * N;
     ext_put<ZPT, K0>(T, b, b + N, F0);
@@ -1019,7 +1048,7 @@ __global__ void __launch_bounds__(256) k_mean(MeanArgs a) {
 // ---------------------------------------------------------------------------
 size_t zpass_lds_bytes(int QPB, int nz) {
     const size_t N = 2 * (size_t)nz;
-    size_t fft = 2 * (size_t)QPB * 4 * N * sizeof(double2);
+    size_t fft = 2 * (size_t)QPB * zpass_lines(QPB, nz) * N * sizeof(double2);
     size_t col = (size_t)QPB * 12 * (nz + 1) * sizeof(double);
     size_t need = fft > col ? fft : col;
     return need + (size_t)QPB * 24 * sizeof(double);

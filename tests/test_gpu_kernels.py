@@ -49,6 +49,9 @@ GRIDS = [
     dict(nx=16, ny=12, nz=128, lower=(0.0, 0.0, -0.5), extent=(2 * PI, 1.5, 2.0)),
     dict(nx=8, ny=16, nz=256, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI)),
     dict(nx=4, ny=6, nz=512, lower=(0.0, 0.0, 0.0), extent=(1.0, 2.0, 3.0)),
+    # long columns on the quad-per-thread kernels (one transform per FFT call)
+    dict(nx=4, ny=4, nz=768, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI)),
+    dict(nx=4, ny=4, nz=1024, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI)),
 ]
 
 

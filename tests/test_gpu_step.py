@@ -202,7 +202,7 @@ def test_derived_pressure_divergence(oracle_mod, G, n, nsteps):
     d.close()
 
 
-@pytest.mark.parametrize("shape", [(16, 12, 128), (8, 16, 256), (4, 6, 512)])
+@pytest.mark.parametrize("shape", [(16, 12, 128), (8, 16, 256), (4, 6, 512), (4, 4, 768), (4, 4, 1024)])
 @pytest.mark.parametrize("filtering", ["Hou & Li", "2/3-rule"])
 def test_long_column_trajectory(oracle_mod, G, shape, filtering):
     """Adaptive steps on grids whose z columns take the wave-per-column
