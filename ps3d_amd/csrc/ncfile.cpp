@@ -1,0 +1,562 @@
+// ncfile.cpp -- ps3d's field and field_stats files in NetCDF classic format
+// (64-bit offset variant, "CDF-2"), written and read without libnetcdf.
+//
+// Field file (src/fields/field_netcdf.f90:65-272, 507-575): dimensions t (unlimited), z
+// (nz+1), y, x; coordinates x, y, z; record variables t and x/y/z_velocity,
+// x/y/z_vorticity and, when requested, pressure and horizontal_divergence, each in
+// C order (t, z, y, x) -- ps3d's transpose-on-write of its (z, y, x) arrays
+// (src/netcdf/netcdf_writer.f90:250-272).  Global attributes: file_type,
+// ncells, extent, origin (netcdf_writer.f90:355-369, 434-443).  ps3d writes
+// NetCDF-4 with a `physical_quantities` group; the classic format has no groups,
+// so those quantities are not written.
+//
+// field_stats file (src/fields/field_diagnostics_netcdf.f90:124-252): dimension t
+// (unlimited) and one record variable per diagnostics value.
+//
+// Format (all integers big-endian): magic "CDF\2", numrecs, dimension list,
+// global attribute list, variable list (name, dimids, attributes, type, vsize,
+// 64-bit begin), non-record data, then records holding every record variable in
+// definition order.
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ps3d_gpu.h"
+#include "../../include/ps3d_nc.h"
+
+namespace {
+
+thread_local std::string nc_err;
+int nfail(const std::string &m) {
+    nc_err = m;
+    return PS3D_E_ARG;
+}
+
+enum { NC_CHAR = 2, NC_INT = 4, NC_DOUBLE = 6 };
+enum { TAG_DIM = 0x0A, TAG_VAR = 0x0B, TAG_ATT = 0x0C };
+
+struct Att {
+    std::string name;
+    int type;
+    std::string text;             // NC_CHAR
+    std::vector<int32_t> ints;    // NC_INT
+    std::vector<double> dbls;     // NC_DOUBLE
+};
+struct Var {
+    std::string name;
+    std::vector<int> dims;
+    std::vector<Att> atts;
+    int type = NC_DOUBLE;
+    uint64_t vsize = 0, begin = 0;
+    bool record = false;
+};
+struct Header {
+    std::vector<std::pair<std::string, uint32_t>> dims;   // length 0 = unlimited
+    std::vector<Att> gatts;
+    std::vector<Var> vars;
+    uint32_t numrecs = 0;
+    uint64_t recsize = 0;
+    uint64_t data_start = 0;
+};
+
+// ---- big-endian buffer ----
+struct Out {
+    std::vector<unsigned char> b;
+    void u32(uint32_t v) {
+        for (int i = 3; i >= 0; --i) b.push_back((unsigned char)(v >> (8 * i)));
+    }
+    void u64(uint64_t v) {
+        for (int i = 7; i >= 0; --i) b.push_back((unsigned char)(v >> (8 * i)));
+    }
+    void f64(double d) {
+        uint64_t v;
+        std::memcpy(&v, &d, 8);
+        u64(v);
+    }
+    void pad() {
+        while (b.size() % 4) b.push_back(0);
+    }
+    void name(const std::string &s) {
+        u32((uint32_t)s.size());
+        b.insert(b.end(), s.begin(), s.end());
+        pad();
+    }
+};
+
+struct In {
+    const unsigned char *p;
+    size_t n, o = 0;
+    bool ok = true;
+    uint32_t u32() {
+        if (o + 4 > n) { ok = false; return 0; }
+        uint32_t v = 0;
+        for (int i = 0; i < 4; ++i) v = (v << 8) | p[o++];
+        return v;
+    }
+    uint64_t u64() {
+        if (o + 8 > n) { ok = false; return 0; }
+        uint64_t v = 0;
+        for (int i = 0; i < 8; ++i) v = (v << 8) | p[o++];
+        return v;
+    }
+    double f64() {
+        uint64_t v = u64();
+        double d;
+        std::memcpy(&d, &v, 8);
+        return d;
+    }
+    void skip_pad() { o = (o + 3) & ~(size_t)3; }
+    std::string name() {
+        const uint32_t len = u32();
+        if (!ok || o + len > n) { ok = false; return {}; }
+        std::string s((const char *)p + o, len);
+        o += len;
+        skip_pad();
+        return s;
+    }
+};
+
+void put_atts(Out &w, const std::vector<Att> &atts) {
+    if (atts.empty()) {
+        w.u32(0);
+        w.u32(0);
+        return;
+    }
+    w.u32(TAG_ATT);
+    w.u32((uint32_t)atts.size());
+    for (const Att &a : atts) {
+        w.name(a.name);
+        w.u32((uint32_t)a.type);
+        if (a.type == NC_CHAR) {
+            w.u32((uint32_t)a.text.size());
+            w.b.insert(w.b.end(), a.text.begin(), a.text.end());
+            w.pad();
+        } else if (a.type == NC_INT) {
+            w.u32((uint32_t)a.ints.size());
+            for (int32_t v : a.ints) w.u32((uint32_t)v);
+        } else {
+            w.u32((uint32_t)a.dbls.size());
+            for (double v : a.dbls) w.f64(v);
+        }
+    }
+}
+
+std::vector<Att> get_atts(In &r) {
+    std::vector<Att> out;
+    const uint32_t tag = r.u32(), n = r.u32();
+    if (tag == 0) return out;
+    if (tag != TAG_ATT) { r.ok = false; return out; }
+    for (uint32_t i = 0; i < n && r.ok; ++i) {
+        Att a;
+        a.name = r.name();
+        a.type = (int)r.u32();
+        const uint32_t m = r.u32();
+        if (a.type == NC_CHAR) {
+            if (r.o + m > r.n) { r.ok = false; break; }
+            a.text.assign((const char *)r.p + r.o, m);
+            r.o += m;
+            r.skip_pad();
+        } else if (a.type == NC_INT) {
+            for (uint32_t k = 0; k < m; ++k) a.ints.push_back((int32_t)r.u32());
+        } else if (a.type == NC_DOUBLE) {
+            for (uint32_t k = 0; k < m; ++k) a.dbls.push_back(r.f64());
+        } else {
+            r.ok = false;
+        }
+        out.push_back(a);
+    }
+    return out;
+}
+
+uint64_t var_bytes(const Header &h, const Var &v) {
+    uint64_t n = 8;   // NC_DOUBLE only
+    for (int d : v.dims)
+        if (h.dims[d].second != 0) n *= h.dims[d].second;
+    return n;
+}
+
+std::vector<unsigned char> encode(Header &h) {
+    // two passes: the header size does not depend on the begin values
+    for (int pass = 0; pass < 2; ++pass) {
+        Out w;
+        w.b = {'C', 'D', 'F', 2};
+        w.u32(h.numrecs);
+        w.u32(TAG_DIM);
+        w.u32((uint32_t)h.dims.size());
+        for (auto &d : h.dims) {
+            w.name(d.first);
+            w.u32(d.second);
+        }
+        put_atts(w, h.gatts);
+        w.u32(TAG_VAR);
+        w.u32((uint32_t)h.vars.size());
+        for (Var &v : h.vars) {
+            w.name(v.name);
+            w.u32((uint32_t)v.dims.size());
+            for (int d : v.dims) w.u32((uint32_t)d);
+            put_atts(w, v.atts);
+            w.u32((uint32_t)v.type);
+            w.u32((uint32_t)(v.vsize > 0xffffffffull ? 0xffffffffu : v.vsize));
+            w.u64(v.begin);
+        }
+        if (pass == 1) return w.b;
+        uint64_t off = w.b.size();
+        h.data_start = off;
+        for (Var &v : h.vars)
+            if (!v.record) {
+                v.vsize = var_bytes(h, v);
+                v.begin = off;
+                off += v.vsize;
+            }
+        h.recsize = 0;
+        for (Var &v : h.vars)
+            if (v.record) {
+                v.vsize = var_bytes(h, v);
+                v.begin = off + h.recsize;
+                h.recsize += v.vsize;
+            }
+    }
+    return {};
+}
+
+int decode(const std::vector<unsigned char> &buf, Header &h) {
+    In r{buf.data(), buf.size()};
+    if (buf.size() < 8 || std::memcmp(buf.data(), "CDF\2", 4) != 0) return nfail("not a NetCDF 64-bit offset file");
+    r.o = 4;
+    h.numrecs = r.u32();
+    uint32_t tag = r.u32(), n = r.u32();
+    if (tag == TAG_DIM)
+        for (uint32_t i = 0; i < n && r.ok; ++i) {
+            std::string nm = r.name();
+            h.dims.push_back({nm, r.u32()});
+        }
+    h.gatts = get_atts(r);
+    tag = r.u32();
+    n = r.u32();
+    if (tag == TAG_VAR)
+        for (uint32_t i = 0; i < n && r.ok; ++i) {
+            Var v;
+            v.name = r.name();
+            const uint32_t nd = r.u32();
+            for (uint32_t k = 0; k < nd; ++k) v.dims.push_back((int)r.u32());
+            v.atts = get_atts(r);
+            v.type = (int)r.u32();
+            v.vsize = r.u32();
+            v.begin = r.u64();
+            for (int d : v.dims)
+                if (d < 0 || d >= (int)h.dims.size()) r.ok = false;
+            v.record = r.ok && !v.dims.empty() && h.dims[v.dims[0]].second == 0;
+            if (r.ok) v.vsize = var_bytes(h, v);
+            h.vars.push_back(v);
+        }
+    if (!r.ok) return nfail("malformed NetCDF header");
+    h.recsize = 0;
+    for (Var &v : h.vars)
+        if (v.record) h.recsize += v.vsize;
+    return 0;
+}
+
+int read_header(const char *path, Header &h) {
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return nfail(std::string("cannot open ") + path);
+    std::vector<unsigned char> buf(1 << 16);
+    const size_t got = std::fread(buf.data(), 1, buf.size(), f);
+    std::fclose(f);
+    buf.resize(got);
+    return decode(buf, h);
+}
+
+Att catt(const char *n, const std::string &s) { return Att{n, NC_CHAR, s, {}, {}}; }
+Att iatt(const char *n, std::vector<int32_t> v) { return Att{n, NC_INT, {}, std::move(v), {}}; }
+Att datt(const char *n, std::vector<double> v) { return Att{n, NC_DOUBLE, {}, {}, std::move(v)}; }
+
+Var rec_var(const char *name, std::vector<int> dims, const char *long_name, const char *unit) {
+    Var v;
+    v.name = name;
+    v.dims = std::move(dims);
+    v.record = true;
+    v.atts = {catt("units", unit), catt("long_name", long_name)};
+    return v;
+}
+
+int write_file(const char *path, Header &h) {
+    std::vector<unsigned char> hdr = encode(h);
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return nfail(std::string("cannot create ") + path);
+    bool ok = std::fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size();
+    std::fclose(f);
+    return ok ? 0 : nfail("write failed");
+}
+
+void be_doubles(const double *src, size_t n, std::vector<unsigned char> &out) {
+    out.resize(n * 8);
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t v;
+        std::memcpy(&v, &src[i], 8);
+        for (int k = 0; k < 8; ++k) out[i * 8 + k] = (unsigned char)(v >> (56 - 8 * k));
+    }
+}
+double from_be(const unsigned char *p) {
+    uint64_t v = 0;
+    for (int k = 0; k < 8; ++k) v = (v << 8) | p[k];
+    double d;
+    std::memcpy(&d, &v, 8);
+    return d;
+}
+
+// append one record: values[k] = data of the k-th record variable (npts doubles each,
+// already in file order)
+int append_record(const char *path, const std::vector<std::vector<double>> &values) {
+    Header h;
+    int st = read_header(path, h);
+    if (st) return st;
+    std::vector<const Var *> rv;
+    for (const Var &v : h.vars)
+        if (v.record) rv.push_back(&v);
+    if (rv.size() != values.size()) return nfail("record layout mismatch");
+    FILE *f = std::fopen(path, "r+b");
+    if (!f) return nfail(std::string("cannot open ") + path);
+    bool ok = true;
+    std::vector<unsigned char> buf;
+    for (size_t k = 0; k < rv.size() && ok; ++k) {
+        if (values[k].size() * 8 != rv[k]->vsize) {
+            std::fclose(f);
+            return nfail("record variable size mismatch: " + rv[k]->name);
+        }
+        const uint64_t off = rv[k]->begin + (uint64_t)h.numrecs * h.recsize;
+        be_doubles(values[k].data(), values[k].size(), buf);
+        ok = fseeko(f, (off_t)off, SEEK_SET) == 0 && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    }
+    if (ok) {
+        const uint32_t nr = h.numrecs + 1;
+        unsigned char b[4] = {(unsigned char)(nr >> 24), (unsigned char)(nr >> 16), (unsigned char)(nr >> 8),
+                              (unsigned char)nr};
+        ok = fseeko(f, 4, SEEK_SET) == 0 && std::fwrite(b, 1, 4, f) == 4;
+    }
+    std::fclose(f);
+    return ok ? 0 : nfail("write failed");
+}
+
+// (x, y, z) host layout -> (z, y, x) file order
+void to_file_order(const double *src, int nx, int ny, int nz1, std::vector<double> &dst) {
+    dst.resize((size_t)nx * ny * nz1);
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iy = 0; iy < ny; ++iy) {
+            const double *c = src + ((size_t)ix * ny + iy) * nz1;
+            for (int iz = 0; iz < nz1; ++iz) dst[((size_t)iz * ny + iy) * nx + ix] = c[iz];
+        }
+}
+
+const char *FIELD_NAMES[8] = {"x_velocity", "y_velocity", "z_velocity", "x_vorticity",
+                              "y_vorticity", "z_vorticity", "pressure", "horizontal_divergence"};
+const char *FIELD_LONG[8] = {"x velocity component", "y velocity component", "z velocity component",
+                             "x vorticity component", "y vorticity component", "z vorticity component",
+                             "pressure", "horizontal divergence"};
+const char *FIELD_UNIT[8] = {"m/s", "m/s", "m/s", "1/s", "1/s", "1/s", "m^2/s^2", "1/s"};
+
+// the diagnostics record (enum ps3d_diag) as field_stats variables: ps3d's name,
+// long name and unit where it has the quantity (field_diagnostics_netcdf.f90:453-798)
+struct StatInfo {
+    const char *name, *long_name, *unit;
+};
+const StatInfo STATS[PS3D_NDIAG] = {
+    {"t", "time ", "seconds since 1970-01-01"},
+    {"dt", "time step", "s"},
+    {"ke", "domain-averaged kinetic energy", "m^2/s^2"},
+    {"en", "domain-averaged enstrophy", "1/s^2"},
+    {"helicity", "domain-averaged helicity", "m/s^2"},
+    {"vortmax", "maximum vorticity magnitude", "1/s"},
+    {"vortrms", "root-mean square vorticity", "1/s"},
+    {"vorch", "characteristic vorticity", "1/s"},
+    {"x_vormean", "mean x-vorticity", "1/s"},
+    {"y_vormean", "mean y-vorticity", "1/s"},
+    {"z_vormean", "mean z-vorticity", "1/s"},
+    {"gmax", "maximum gamma", "1/s"},
+    {"umax", "maximum x-velocity", "m/s"},
+    {"vmax", "maximum y-velocity", "m/s"},
+    {"wmax", "maximum z-velocity", "m/s"},
+    {"usgmax", "upper surface maximum strain", "1/s"},
+    {"lsgmax", "lower surface maximum strain", "1/s"},
+    {"rolling_mean_gmax", "rolling mean maximum gamma", "1/s"},
+    {"prefactor", "diffusion prefactor", "1"},
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *ps3d_nc_last_error(void) { return nc_err.c_str(); }
+
+int ps3d_nc_create_fields(const char *path, int nx, int ny, int nz, const double *lower, const double *extent,
+                          int with_derived) {
+    if (!path || !lower || !extent || nx < 1 || ny < 1 || nz < 1) return nfail("invalid argument");
+    const uint64_t vol = 8ull * nx * ny * (nz + 1);
+    if (vol > 0xffffffffull) return nfail("a field record exceeds 4 GiB: too large for the 64-bit offset format");
+    Header h;
+    h.dims = {{"t", 0}, {"z", (uint32_t)(nz + 1)}, {"y", (uint32_t)ny}, {"x", (uint32_t)nx}};
+    h.gatts = {catt("file_type", "fields"), catt("Conventions", "CF-1.9"),
+               iatt("ncells", {nx, ny, nz}), datt("extent", {extent[0], extent[1], extent[2]}),
+               datt("origin", {lower[0], lower[1], lower[2]})};
+    const char *axn[3] = {"x", "y", "z"};
+    const int axd[3] = {3, 2, 1};
+    const char *axu[3] = {"X", "Y", "Z"};
+    for (int a = 0; a < 3; ++a) {
+        Var v;
+        v.name = axn[a];
+        v.dims = {axd[a]};
+        v.atts = {catt("units", "m"),
+                  catt("long_name", a < 2 ? std::string(axn[a]) + "-coordinate in projected coordinate system"
+                                          : std::string("height coordinate in projected coordinate system")),
+                  catt("standard_name", a < 2 ? std::string("projection_") + axn[a] + "_coordinate"
+                                              : std::string("height")),
+                  catt("axis", axu[a])};
+        h.vars.push_back(v);
+    }
+    Var tv = rec_var("t", {0}, "time ", "seconds since 1970-01-01");
+    tv.atts.push_back(catt("standard_name", "time"));
+    tv.atts.push_back(catt("axis", "T"));
+    h.vars.push_back(tv);
+    for (int k = 0; k < (with_derived ? 8 : 6); ++k) h.vars.push_back(rec_var(FIELD_NAMES[k], {0, 1, 2, 3}, FIELD_LONG[k], FIELD_UNIT[k]));
+    int st = write_file(path, h);
+    if (st) return st;
+    // coordinates x_i = lower + i dx (periodic, no end point), z_k = lower + k dz
+    FILE *f = std::fopen(path, "r+b");
+    if (!f) return nfail(std::string("cannot open ") + path);
+    const int n[3] = {nx, ny, nz};
+    bool ok = true;
+    std::vector<unsigned char> buf;
+    for (int a = 0; a < 3 && ok; ++a) {
+        const int m = a < 2 ? n[a] : nz + 1;
+        std::vector<double> c(m);
+        const double d = extent[a] / (double)n[a];
+        for (int i = 0; i < m; ++i) c[i] = lower[a] + d * (double)i;
+        be_doubles(c.data(), c.size(), buf);
+        ok = fseeko(f, (off_t)h.vars[a].begin, SEEK_SET) == 0 && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    }
+    std::fclose(f);
+    return ok ? 0 : nfail("write failed");
+}
+
+int ps3d_nc_append_fields(const char *path, double t, int nx, int ny, int nz, const double *vel, const double *vor,
+                          const double *pres, const double *delta) {
+    if (!path || !vel || !vor) return nfail("invalid argument");
+    Header h;
+    int st = read_header(path, h);
+    if (st) return st;
+    int nrec = 0;
+    for (const Var &v : h.vars) nrec += v.record;
+    const bool derived = nrec == 9;
+    if (derived && (!pres || !delta)) return nfail("this field file holds pressure and divergence: pass them");
+    const size_t hv = (size_t)nx * ny * (nz + 1);
+    std::vector<std::vector<double>> vals;
+    vals.push_back({t});
+    const double *src[8] = {vel, vel + hv, vel + 2 * hv, vor, vor + hv, vor + 2 * hv, pres, delta};
+    for (int k = 0; k < (derived ? 8 : 6); ++k) {
+        vals.emplace_back();
+        to_file_order(src[k], nx, ny, nz + 1, vals.back());
+    }
+    return append_record(path, vals);
+}
+
+int ps3d_nc_read_fields(const char *path, int step, double *t, int nx, int ny, int nz, double *vel, double *vor) {
+    if (!path || !vor) return nfail("invalid argument");
+    Header h;
+    int st = read_header(path, h);
+    if (st) return st;
+    if (h.numrecs == 0) return nfail("no records");
+    // field_netcdf.f90:286-299: step counts from 1; -1 or a step past the end reads the last
+    if (step == 0 || step < -1) return nfail("step must be >= 1 or -1");
+    const int rec = (step == -1 || step > (int)h.numrecs) ? (int)h.numrecs - 1 : step - 1;
+    auto dim = [&](const char *n) -> int {
+        for (auto &d : h.dims)
+            if (d.first == n) return (int)d.second;
+        return -1;
+    };
+    if (dim("x") != nx || dim("y") != ny || dim("z") != nz + 1) return nfail("grid differs from the file's");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return nfail(std::string("cannot open ") + path);
+    const size_t hv = (size_t)nx * ny * (nz + 1);
+    std::vector<unsigned char> buf;
+    bool ok = true;
+    auto read_var = [&](const char *name, double *dst, bool scalar) {
+        for (const Var &v : h.vars) {
+            if (v.name != name || !v.record) continue;
+            buf.resize(v.vsize);
+            const uint64_t off = v.begin + (uint64_t)rec * h.recsize;
+            ok = ok && fseeko(f, (off_t)off, SEEK_SET) == 0 && std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+            if (!ok) return false;
+            if (scalar) {
+                *dst = from_be(buf.data());
+            } else {
+                for (int iz = 0; iz <= nz; ++iz)
+                    for (int iy = 0; iy < ny; ++iy)
+                        for (int ix = 0; ix < nx; ++ix)
+                            dst[((size_t)ix * ny + iy) * (nz + 1) + iz] =
+                                from_be(&buf[(((size_t)iz * ny + iy) * nx + ix) * 8]);
+            }
+            return true;
+        }
+        return false;
+    };
+    double tt = 0.0;
+    bool found = read_var("t", &tt, true);
+    for (int c = 0; c < 3 && found; ++c) found = read_var(FIELD_NAMES[3 + c], vor + c * hv, false);
+    for (int c = 0; c < 3 && found && vel; ++c) found = read_var(FIELD_NAMES[c], vel + c * hv, false);
+    std::fclose(f);
+    if (!ok) return nfail("read failed");
+    if (!found) return nfail("field file lacks a vorticity or velocity component");
+    if (t) *t = tt;
+    return 0;
+}
+
+int ps3d_nc_records(const char *path, int *nrecords, double *t_last) {
+    if (!path || !nrecords) return nfail("invalid argument");
+    Header h;
+    int st = read_header(path, h);
+    if (st) return st;
+    *nrecords = (int)h.numrecs;
+    if (!t_last) return 0;
+    *t_last = 0.0;
+    for (const Var &v : h.vars) {
+        if (v.name != "t" || !v.record || h.numrecs == 0) continue;
+        FILE *f = std::fopen(path, "rb");
+        if (!f) return nfail(std::string("cannot open ") + path);
+        unsigned char b[8];
+        const uint64_t off = v.begin + (uint64_t)(h.numrecs - 1) * h.recsize;
+        const bool ok = fseeko(f, (off_t)off, SEEK_SET) == 0 && std::fread(b, 1, 8, f) == 8;
+        std::fclose(f);
+        if (!ok) return nfail("read failed");
+        *t_last = from_be(b);
+    }
+    return 0;
+}
+
+int ps3d_nc_create_stats(const char *path) {
+    if (!path) return nfail("invalid argument");
+    Header h;
+    h.dims = {{"t", 0}};
+    h.gatts = {catt("file_type", "field_stats"), catt("Conventions", "CF-1.9")};
+    for (int k = 0; k < PS3D_NDIAG; ++k) {
+        h.vars.push_back(rec_var(STATS[k].name, {0}, STATS[k].long_name, STATS[k].unit));
+        if (k == 0) {
+            h.vars.back().atts.push_back(catt("standard_name", "time"));
+            h.vars.back().atts.push_back(catt("axis", "T"));
+        }
+    }
+    return write_file(path, h);
+}
+
+int ps3d_nc_append_stats(const char *path, const double *rows, int nrows) {
+    if (!path || !rows || nrows < 0) return nfail("invalid argument");
+    for (int r = 0; r < nrows; ++r) {
+        std::vector<std::vector<double>> vals;
+        for (int k = 0; k < PS3D_NDIAG; ++k) vals.push_back({rows[(size_t)r * PS3D_NDIAG + k]});
+        int st = append_record(path, vals);
+        if (st) return st;
+    }
+    return 0;
+}
+
+}  // extern "C"

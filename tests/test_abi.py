@@ -1,5 +1,5 @@
 """The C-ABI library builds for gfx950, loads without a GPU and exports every
-entry point include/ps3d_gpu.h declares (no compute calls here)."""
+entry point the headers under include/ declare (no compute calls here)."""
 import ctypes
 import os
 import re
@@ -10,15 +10,20 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    src = open(os.path.join(ROOT, "include", "ps3d_gpu.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(ps3d_(?:gpu_\w+|params_default))\s*\(", src)))
+    inc = os.path.join(ROOT, "include")
+    syms = set()
+    for name in sorted(os.listdir(inc)):
+        if name.endswith(".h"):
+            src = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, name)).read(), flags=re.S)
+            syms.update(re.findall(r"\b(ps3d_\w+)\s*\(", src))
+    return sorted(syms)
 
 
 def test_header_declares_entry_points():
     syms = declared_symbols()
     for s in ("ps3d_gpu_create", "ps3d_gpu_vor2vel", "ps3d_gpu_source", "ps3d_gpu_adapt",
-              "ps3d_gpu_step", "ps3d_gpu_advance_n", "ps3d_gpu_op", "ps3d_gpu_fft1d"):
+              "ps3d_gpu_step", "ps3d_gpu_advance_n", "ps3d_gpu_op", "ps3d_gpu_fft1d",
+              "ps3d_cnn_forward", "ps3d_nc_create_fields", "ps3d_nc_append_stats"):
         assert s in syms
 
 
