@@ -196,9 +196,12 @@ struct LineArgs {
     Batch bt;
     uint16_t *nn_in;   // LM_XMAIN: also write the closure input (bf16 [p][16]) when set
     float nn_scale[6];
+    int rthreads;      // register-holding modes: 1024 (standard tile) or 512 (half depth)
 };
 
 size_t line_lds_bytes(int n, int ZC);
+// whether the 512-thread register build exists for line length n
+bool line_reg_half_ok(int n);
 hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream);
 
 // Reductions: finishes the Parseval sums (after LM_INV_Y1) and the adapt step

@@ -24,6 +24,11 @@ struct Tile {
 __host__ __device__ constexpr int zc_for(int n) {
     return ((4096 / n > 16 ? 16 : (4096 / n < 2 ? 2 : 4096 / n)) & ~1);
 }
+// z depth of the register-holding modes at T threads per block: the 1024-thread
+// build takes the standard tile, the 512-thread build half of it (at least 2)
+__host__ __device__ constexpr int zc_reg(int n, int T) {
+    return T >= kLineThreads ? zc_for(n) : ((zc_for(n) / 2) < 2 ? 2 : ((zc_for(n) / 2) & ~1));
+}
 
 // Register-holding modes run 1024 threads per tile and use the radix-8 plans to
 // stay within 128 VGPRs; transform-only modes use the radix-16 plans.
@@ -228,37 +233,37 @@ __device__ inline void nn_export(const Tile &t, const LineArgs &a, const double2
         }
 }
 
-// Register prefetch of a tile for the 1024-thread kernels: the global loads of the
-// next field are issued before the transform of the current one.
-template <int N>
+// Register prefetch of a tile for the register-holding kernels (T threads): the
+// global loads of the next field are issued before the transform of the current one.
+template <int N, int T = kLineThreads>
 struct Pf {
-    static constexpr int ZC = N > 0 ? zc_for(N) : 2;
-    static constexpr int PER = N > 0 ? (N * ZC + kLineThreads - 1) / kLineThreads : 1;
+    static constexpr int ZC = N > 0 ? zc_reg(N, T) : 2;
+    static constexpr int PER = N > 0 ? (N * ZC + T - 1) / T : 1;
     double pre[PER];
 };
-template <int N>
-__device__ inline void pf_load(Pf<N> &p, const Tile &t, const LineArgs &a, const double *__restrict__ src) {
-    constexpr int ZC = Pf<N>::ZC;
+template <int N, int T>
+__device__ inline void pf_load(Pf<N, T> &p, const Tile &t, const LineArgs &a, const double *__restrict__ src) {
+    constexpr int ZC = Pf<N, T>::ZC;
 #pragma unroll
-    for (int k = 0; k < Pf<N>::PER; ++k) {
-        const int e = threadIdx.x + k * kLineThreads, zz = e % ZC, j = e / ZC;
+    for (int k = 0; k < Pf<N, T>::PER; ++k) {
+        const int e = threadIdx.x + k * T, zz = e % ZC, j = e / ZC;
         p.pre[k] = (j < t.n && zz < t.zce) ? src[t.base + (long)j * a.estride + zz] : 0.0;
     }
 }
-template <int N>
-__device__ inline void pf_commit(const Pf<N> &p, const Tile &t, double2 *A) {
-    constexpr int ZC = Pf<N>::ZC;
+template <int N, int T>
+__device__ inline void pf_commit(const Pf<N, T> &p, const Tile &t, double2 *A) {
+    constexpr int ZC = Pf<N, T>::ZC;
     double *d = reinterpret_cast<double *>(A);
 #pragma unroll
-    for (int k = 0; k < Pf<N>::PER; ++k) {
-        const int e = threadIdx.x + k * kLineThreads, zz = e % ZC, j = e / ZC;
+    for (int k = 0; k < Pf<N, T>::PER; ++k) {
+        const int e = threadIdx.x + k * T, zz = e % ZC, j = e / ZC;
         if (j < t.n) d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = p.pre[k];
     }
 }
 // field_phys on the prefetched tile (spectral input; diff: x/y derivative first),
 // starting the loads of `next` (nullptr: none).  N = 0 falls back to field_phys.
-template <int N>
-__device__ inline void phys_pf(Pf<N> &pf, const Tile &t, const LineArgs &a, double2 *A, double2 *B,
+template <int N, int T>
+__device__ inline void phys_pf(Pf<N, T> &pf, const Tile &t, const LineArgs &a, double2 *A, double2 *B,
                                double2 (&r)[kEpt], bool diff, const double *cur, const double *next) {
     if constexpr (N == 0) {
         field_phys<N>(t, a, cur, A, B, r, diff);
@@ -427,8 +432,11 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
 
 // Modes that keep physical values in registers (products, adapt diagnostics); one
 // instantiation per mode M so that each gets its own register allocation.
-template <int N, int M>
-__global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
+// T = 1024: one block per CU; T = 512 (half-depth tiles, at most 128 VGPRs): two
+// blocks per CU, so that one block's barriers overlap the other's loads.
+template <int N, int M, int T>
+__global__ void __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= kLineThreads ? 1 : 4)))
+k_line_reg(LineArgs a) {
     extern __shared__ double2 lsmem[];
     Tile t = make_tile(a);
     const size_t tb = (size_t)t.nl * t.LS;
@@ -453,7 +461,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         double suu = 0.0, sww = 0.0, suw = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
         // every field is loaded through the register prefetch: the loads of the
         // next source are in flight during the current transform
-        Pf<N> pf;
+        Pf<N, T> pf;
         auto take = [&](const double *cur, const double *next) {
             if constexpr (N > 0) {
                 pf_commit(pf, t, A);
@@ -539,7 +547,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         if constexpr (N > 0) {
             Tile t2 = t;
             t2.nl = 2 * t.nl;
-            Pf<N> p0, p1;
+            Pf<N, T> p0, p1;
             auto phys2 = [&](int f, double2(&r0)[kEpt], double2(&r1)[kEpt]) {
                 pf_commit(p0, t, A);
                 pf_commit(p1, t, A + tb);
@@ -635,7 +643,7 @@ __global__ void __launch_bounds__(kLineThreads) k_line_reg(LineArgs a) {
         // Fields in pairs (16 lines per FFT call): (u, v) (w, xi) (u_x, w_x) (u_y, v_y)
         // (w_y, eta) (zeta) then the midpoint vorticity (mxi, meta) (mzeta); spectral
         // inputs prefetch the next pair during the current transform.
-        Pf<N> p0, p1;
+        Pf<N, T> p0, p1;
         Tile t2 = t;
         t2.nl = 2 * t.nl;
         auto get2 = [&](int fa, int fb, bool diff, double2(&ra)[kEpt], double2(&rb)[kEpt], int na, int nb) {
@@ -753,15 +761,30 @@ size_t line_lds_bytes(int n, int ZC) {
 hipError_t configure_linepass() {
     const int lds = 160 * 1024;
     hipError_t e = hipSuccess;
+#define CFGT(N_, T_)                                                                                         \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_INV_Y1, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XMAIN, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XADAPT, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 #define CFG(N_)                                                                                              \
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_simple<N_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_INV_Y1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XMAIN>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XADAPT>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    CFGT(N_, kLineThreads)
+#define CFG2(N_) CFG(N_) CFGT(N_, 512)
     CFG(0)
-    PS3D_LINE_SIZES(CFG)
+    PS3D_LINE_SIZES(CFG2)
+#undef CFG2
 #undef CFG
+#undef CFGT
     return e;
+}
+
+template <int N, int T>
+static void launch_reg_t(const LineArgs &a, hipStream_t stream, int blocks, int nb, size_t lds) {
+    if (a.mode == LM_INV_Y1)
+        hipLaunchKernelGGL((k_line_reg<N, LM_INV_Y1, T>), dim3(blocks, nb), dim3(T), lds, stream, a);
+    else if (a.mode == LM_XMAIN)
+        hipLaunchKernelGGL((k_line_reg<N, LM_XMAIN, T>), dim3(blocks, nb), dim3(T), lds, stream, a);
+    else
+        hipLaunchKernelGGL((k_line_reg<N, LM_XADAPT, T>), dim3(blocks, nb), dim3(T), lds, stream, a);
 }
 
 template <int N>
@@ -773,18 +796,27 @@ static void launch_line_t(const LineArgs &a, hipStream_t stream) {
         lds += (size_t)2 * (a.ZC / 2) * (a.n + 1) * sizeof(double2);   // field pairs
     if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF)
         hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks, nb), dim3(256), lds, stream, a);
-    else if (a.mode == LM_INV_Y1)
-        hipLaunchKernelGGL((k_line_reg<N, LM_INV_Y1>), dim3(blocks, nb), dim3(kLineThreads), lds, stream, a);
-    else if (a.mode == LM_XMAIN)
-        hipLaunchKernelGGL((k_line_reg<N, LM_XMAIN>), dim3(blocks, nb), dim3(kLineThreads), lds, stream, a);
+    else if (a.rthreads == 512 && N > 0)
+        launch_reg_t<N, 512>(a, stream, blocks, nb, lds);
     else
-        hipLaunchKernelGGL((k_line_reg<N, LM_XADAPT>), dim3(blocks, nb), dim3(kLineThreads), lds, stream, a);
+        launch_reg_t<N, kLineThreads>(a, stream, blocks, nb, lds);
+}
+
+bool line_reg_half_ok(int n) {
+    switch (n) {
+#define CASE(N_) case N_: return true;
+        PS3D_LINE_SIZES(CASE)
+#undef CASE
+        default: return false;
+    }
 }
 
 hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
     (void)nthreads;
-    // compile-time plans need the standard tile depth
-    const bool std_zc = a.ZC == zc_for(a.n);
+    // compile-time plans need the standard tile depth of their build
+    const bool reg = a.mode == LM_INV_Y1 || a.mode == LM_XMAIN || a.mode == LM_XADAPT;
+    const bool std_zc = a.ZC == (reg ? zc_reg(a.n, a.rthreads == 512 ? 512 : kLineThreads) : zc_for(a.n));
+    if (reg && a.rthreads == 512 && !std_zc) return hipErrorInvalidValue;
     switch (std_zc ? a.n : 0) {
 #define CASE(N_) case N_: launch_line_t<N_>(a, stream); break;
         PS3D_LINE_SIZES(CASE)

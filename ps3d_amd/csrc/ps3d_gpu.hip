@@ -64,6 +64,11 @@ struct ps3d_handle {
     ZLaunch zl{};
     int TQ = 0, QPB = 0, nquads = 0;
     LineCfg lx{}, ly{};
+    // tiles of the register-holding line modes (x products pass lxm, x adapt pass lxr,
+    // adapt-stage y pass lyr) and their block sizes: 1024 threads on the standard
+    // tile, or 512 on half of it (two blocks per CU)
+    LineCfg lxm{}, lxr{}, lyr{};
+    int rtm = 1024, rtx = 1024, rty = 1024;
     double visc = 0.0;
     bool have_state = false, have_phys = false, have_adapt = false;
     // batched instances of this grid (ensemble group): instance b's data sit at
@@ -151,13 +156,14 @@ static bool factor23(int n) {
     return n == 1;
 }
 
-static LineCfg line_cfg(int n, int nz, int zs, int nother, long estride, long ostride) {
+static LineCfg line_cfg(int n, int nz, int zs, int nother, long estride, long ostride, bool half = false) {
     LineCfg c;
     c.n = n;
     int zc = 4096 / n;
     if (zc > 16) zc = 16;
     if (zc < 2) zc = 2;
     zc &= ~1;
+    if (half) zc = std::max(2, (zc / 2) & ~1);
     c.ZC = zc;
     c.nzt = (nz + 1 + zc - 1) / zc;
     c.nother = nother;
@@ -317,14 +323,30 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
 
     h->ly = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs);
     h->lx = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs);
+    {
+        // 512-thread half-depth tiles (bit 0: x products pass, bit 1: adapt-stage y
+        // pass, bit 2: x adapt pass).  Measured at 256^3 all three gain; at 512^3 only
+        // the x adapt pass does.  PS3D_LINE512 overrides the choice.
+        const char *ev = std::getenv("PS3D_LINE512");
+        const int half = ev ? std::atoi(ev) : (std::max(nx, ny) <= 256 ? 7 : 4);
+        const bool hm = (half & 1) && ps3d::line_reg_half_ok(nx);
+        const bool hy = (half & 2) && ps3d::line_reg_half_ok(ny);
+        const bool hx = (half & 4) && ps3d::line_reg_half_ok(nx);
+        h->lxm = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, hm);
+        h->lyr = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs, hy);
+        h->lxr = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, hx);
+        h->rtm = hm ? 512 : 1024;
+        h->rty = hy ? 512 : 1024;
+        h->rtx = hx ? 512 : 1024;
+    }
 
     HIPCHK(ps3d::configure_zpass());
     HIPCHK(ps3d::configure_zwave());
     HIPCHK(ps3d::configure_linepass());
 
     const size_t nb = (size_t)nbatch;
-    h->b_party = (long)h->ly.nother * h->ly.nzt * RY_N;
-    h->b_partx = (long)h->lx.nother * h->lx.nzt * RX_N;
+    h->b_party = (long)h->lyr.nother * h->lyr.nzt * RY_N;
+    h->b_partx = (long)h->lxr.nother * h->lxr.nzt * RX_N;
     h->b_roll = std::max(1, p->roll_mean_win_size);
     HIPCHK(hipMalloc((void **)&h->scal, sizeof(double) * SC_NSCAL * nb));
     HIPCHK(hipMemset(h->scal, 0, sizeof(double) * SC_NSCAL * nb));
@@ -365,7 +387,9 @@ static LineArgs line_args(const ps3d_handle *h, int axis, int mode) {
     LineArgs a;
     std::memset(&a, 0, sizeof a);
     a.bt = batch(h);
-    const LineCfg &c = axis == 0 ? h->lx : h->ly;
+    const LineCfg &c = mode == LM_XMAIN ? h->lxm : mode == LM_XADAPT ? h->lxr : mode == LM_INV_Y1 ? h->lyr
+                     : (axis == 0 ? h->lx : h->ly);
+    a.rthreads = mode == LM_XMAIN ? h->rtm : mode == LM_XADAPT ? h->rtx : mode == LM_INV_Y1 ? h->rty : 256;
     a.g = h->g;
     a.axis = axis;
     a.n = c.n;
@@ -405,9 +429,9 @@ static FinArgs fin_args(ps3d_handle *h) {
     FinArgs f;
     std::memset(&f, 0, sizeof f);
     f.party = h->party;
-    f.nby = h->ly.nother * h->ly.nzt;
+    f.nby = h->lyr.nother * h->lyr.nzt;
     f.partx = h->partx;
-    f.nbx = h->lx.nother * h->lx.nzt;
+    f.nbx = h->lxr.nother * h->lxr.nzt;
     f.scal = h->scal;
     f.diag = h->diag;
     f.max_rows = h->diag_cap;
@@ -708,6 +732,8 @@ static ps3d_handle make_view(const ps3d_handle *h, int b) {
     v.zl = h->zl;
     v.TQ = h->TQ; v.QPB = h->QPB; v.nquads = h->nquads;
     v.lx = h->lx; v.ly = h->ly;
+    v.lxm = h->lxm; v.lxr = h->lxr; v.lyr = h->lyr;
+    v.rtm = h->rtm; v.rtx = h->rtx; v.rty = h->rty;
     v.h_rkx = h->h_rkx;
     v.h_rky = h->h_rky;
     v.nbatch = 1;
