@@ -42,6 +42,17 @@ __device__ inline double2 *lfft(const Tile &t, const LineArgs &a, double2 *A, do
         return cfft_lds<N>(A, B, t.nl, t.LS, t.twl, dir);
 }
 
+// Tile geometry fixed at compile time for a built line length: the index arithmetic
+// of the loads, stores and transforms then divides by constants.
+__device__ inline void fix_tile(Tile &t, int n, int zc) {
+    t.n = n;
+    t.ZC = zc;
+    t.nl = zc / 2;
+    t.LS = n + 1;
+    const int rem = t.zce;
+    t.zce = rem < zc ? rem : zc;
+}
+
 __device__ inline Tile make_tile(const LineArgs &a) {
     Tile t;
     t.n = a.n;
@@ -373,6 +384,7 @@ template <int N>
 __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
     extern __shared__ double2 lsmem[];
     Tile t = make_tile(a);
+    if constexpr (N > 0) fix_tile(t, N, zc_for(N));
     const size_t tb = (size_t)t.nl * t.LS;
     double2 *A = lsmem, *B = lsmem + tb;
     if constexpr (N > 0) {
@@ -439,6 +451,7 @@ __global__ void __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= kLi
 k_line_reg(LineArgs a) {
     extern __shared__ double2 lsmem[];
     Tile t = make_tile(a);
+    if constexpr (N > 0) fix_tile(t, N, zc_reg(N, T));
     const size_t tb = (size_t)t.nl * t.LS;
     // the products pass transforms field pairs: A and B hold two tiles each
     constexpr bool PAIRS = ((M == LM_XMAIN || M == LM_XADAPT) && N > 0);
