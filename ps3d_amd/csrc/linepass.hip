@@ -378,6 +378,27 @@ __device__ inline double jacobi_absmax(double s11, double s12, double s13, doubl
     return fmax(fmax(fabs(D[0]), fabs(D[1])), fabs(D[2]));
 }
 
+// Largest |eigenvalue| of a symmetric 3x3 matrix from the trigonometric solution of
+// its characteristic cubic (Smith, CACM 4(4), 1961).  Within a few 1e-8 of the exact
+// value relative to the matrix norm (acos loses half the digits next to a double
+// root), so it only selects the points whose Jacobi value can be a block maximum.
+__device__ inline double eig_absmax_est(double s11, double s12, double s13, double s22, double s23, double s33) {
+    const double p1 = s12 * s12 + s13 * s13 + s23 * s23;
+    const double q = (s11 + s22 + s33) * (1.0 / 3.0);
+    const double d1 = s11 - q, d2 = s22 - q, d3 = s33 - q;
+    const double p2 = d1 * d1 + d2 * d2 + d3 * d3 + 2.0 * p1;
+    if (!(p2 > 0.0)) return fabs(q);
+    const double p = sqrt(p2 * (1.0 / 6.0));
+    const double det = d1 * (d2 * d3 - s23 * s23) - s12 * (s12 * d3 - s23 * s13) + s13 * (s12 * s23 - d2 * s13);
+    const double ip = 1.0 / p;
+    const double r = fmin(1.0, fmax(-1.0, 0.5 * det * ip * ip * ip));
+    const double c = cos(acos(r) * (1.0 / 3.0));            // angle in [0, pi/3]
+    const double sn = sqrt(fmax(0.0, 1.0 - c * c));
+    const double l1 = q + 2.0 * p * c;                        // largest eigenvalue
+    const double l3 = q - p * (c + 1.7320508075688772 * sn);  // smallest eigenvalue
+    return fmax(fabs(l1), fabs(l3));
+}
+
 // Transform-only modes (forward, inverse, derivative) of nf fields; the global
 // loads of field f+1 are issued before the FFT of field f.
 template <int N>
@@ -705,22 +726,41 @@ k_line_reg(LineArgs a) {
         get2(6, 7, false, uy, vy, 8, 4);       // du/dy, dv/dy
         get2(8, 4, false, wy, et, 5, -1);      // dw/dy, eta
         get2(5, -1, false, ze, ze, 9, 10);     // zeta
+        // S11 = u_x, S12 = u_y + zeta/2, S13 = w_x + eta/2, S22 = v_y,
+        // S23 = w_y - xi/2, S33 = -(u_x + v_y)  (advance.f90:225-257).  The block's
+        // maximum of the Jacobi values is found by running Jacobi only where the
+        // closed-form estimate comes within 1e-6 of the block's largest estimate (the
+        // estimate's error is far below that margin, so the maximising point is always
+        // among them); lid points, which have maxima of their own, always run it.
+#define C_(V) (h ? (V).y : (V).x)
+#define STRAIN_ARGS C_(ux[k]), C_(uy[k]) + 0.5 * C_(ze[k]), C_(wx[k]) + 0.5 * C_(et[k]), C_(vy[k]), \
+                    C_(wy[k]) - 0.5 * C_(xi[k]), -(C_(ux[k]) + C_(vy[k]))
+        float est[kEpt][2];   // float keeps the registers down; its rounding is far below the margin
+        float emax = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kEpt; ++k)
+            for (int h = 0; h < 2; ++h) {
+                est[k][h] = -1.0f;
+                if (!reg_valid(t, k, h, nz)) continue;
+                const double o1 = C_(xi[k]), o2 = C_(et[k]), o3 = C_(ze[k]);
+                w2max = fmax(w2max, o1 * o1 + o2 * o2 + o3 * o3);
+                est[k][h] = (float)eig_absmax_est(STRAIN_ARGS);
+                emax = fmaxf(emax, est[k][h]);
+            }
+        const float ecut = (float)(block_max((double)emax, scratch) * (1.0 - 1.0e-6));
 #pragma unroll
         for (int k = 0; k < kEpt; ++k)
             for (int h = 0; h < 2; ++h) {
                 if (!reg_valid(t, k, h, nz)) continue;
                 const int z = reg_z(t, k, h);
-#define C_(V) (h ? (V).y : (V).x)
-                const double o1 = C_(xi[k]), o2 = C_(et[k]), o3 = C_(ze[k]);
-                w2max = fmax(w2max, o1 * o1 + o2 * o2 + o3 * o3);
-                // S11 = u_x, S12 = u_y + zeta/2, S13 = w_x + eta/2, S22 = v_y,
-                // S23 = w_y - xi/2, S33 = -(u_x + v_y)  (advance.f90:225-257)
-                const double lmax = jacobi_absmax(C_(ux[k]), C_(uy[k]) + 0.5 * o3, C_(wx[k]) + 0.5 * o2, C_(vy[k]),
-                                                  C_(wy[k]) - 0.5 * o1, -(C_(ux[k]) + C_(vy[k])));
+                const bool lid = z == 0 || z == nz;
+                if (!lid && !(est[k][h] >= ecut)) continue;
+                const double lmax = jacobi_absmax(STRAIN_ARGS);
                 gg = fmax(gg, lmax);
                 if (z == nz) usgg = fmax(usgg, lmax);
                 if (z == 0) lsgg = fmax(lsgg, lmax);
             }
+#undef STRAIN_ARGS
         // characteristic vorticity <|w|^2>/<|w|> over cells with |w| > w_rms
         // (field_diagnostics.f90:501-545), from the z-midpoint vorticity
         get2(9, 10, false, xi, et, 11, -1);
