@@ -315,8 +315,9 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     {
         const char *ev = std::getenv("PS3D_ZWAVE");
         const bool off = ev && ev[0] == '0';
-        // register variant: at nz = 512 the three-waves-per-SIMD build spills heavily
-        int var = nz == 512 ? 2 : 1;
+        // register variant: the two-waves-per-SIMD builds (no spills, the update's
+        // operands loaded ahead) measure fastest at nz = 256 and 512
+        int var = 2;
         if (ev && ev[0] >= '1' && ev[0] <= '3') var = ev[0] - '0';
         h->zl.wave = (ps3d::zwave_supported(nz) && !off) ? var : 0;
     }

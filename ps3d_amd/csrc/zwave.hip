@@ -386,81 +386,104 @@ __device__ inline void xget(const WC &c, double2 *wsm, int member, int field, WF
 }
 
 // RK4 / cn2 update of this wave's column from the tendency S (same formulas as the
-// quad kernels: impl_rk4.f90:212-364, cn2.f90:92-181).
+// quad kernels: impl_rk4.f90:212-364, cn2.f90:92-181).  The stored operands of one
+// component are loaded ahead of its update (wrk_load), so that their latency overlaps
+// the partner exchange and the previous component's update.
 template <int NZ>
-__device__ inline void wrk_update(const WC &c, const RkArgs &rk, int mode, int stage, const WF<NZ> (&S)[3],
-                                  const ZTabs &tb) {
+struct WOps {
+    WF<NZ> a, b;
+};
+template <int NZ>
+__device__ __attribute__((always_inline)) inline void wrk_load(const WC &c, const RkArgs &rk, int mode, int stage, int cc, WOps<NZ> &o) {
     if (!c.store) return;
-    constexpr int NS = WDim<NZ>::S;
+    // fixed destinations (a: q, qdi or qdf; b: qdf at RK4 stages 2 and 3)
+    const bool two = mode != ZT_CN2 && (stage == 2 || stage == 3);
+    const double *pa = (stage == 1) ? rk.q[cc] : (two ? rk.qdi[cc] : rk.qdf[cc]);
+    wload(c, pa, o.a);
+    if (two) wload(c, rk.qdf[cc], o.b);
+}
+
+struct WRk {
+    double dt, f0, vd, em, epf;
+};
+template <int NZ>
+__device__ inline WRk wrk_setup(const WC &c, const RkArgs &rk, int mode, const ZTabs &tb) {
+    WRk r;
     const double *sc = rk.scal + (size_t)blockIdx.y * SC_NSCAL;
-    const double dt = sc[SC_DT];
-    double f0;
+    r.dt = sc[SC_DT];
     if (c.q00)
-        f0 = 1.0;
+        r.f0 = 1.0;
     else if (tb.hou_li)
-        f0 = exp(tb.filx[c.kx] + tb.fily[c.ky]);
+        r.f0 = exp(tb.filx[c.kx] + tb.fily[c.ky]);
     else
-        f0 = tb.filx[c.kx] * tb.fily[c.ky];
+        r.f0 = tb.filx[c.kx] * tb.fily[c.ky];
     double kp = 1.0;
     for (int i = 0; i < tb.nnu; ++i) kp *= c.k2l2;
     const double vhdis = sc[SC_VISC] * kp;
-    const int l = c.lane;
     if (mode == ZT_CN2) {
-        const double vd = 1.0 / (1.0 + sc[SC_DFAC] * vhdis);
+        r.vd = 1.0 / (1.0 + sc[SC_DFAC] * vhdis);
+        r.em = r.epf = 0.0;
+    } else {
+        const double D = rk.vdiss ? rk.vdiss[c.colidx] : sc[SC_DFAC] * vhdis;
+        const double ep = exp(D);
+        r.em = 1.0 / ep;
+        r.epf = ep * r.f0;
+        r.vd = 0.0;
+    }
+    return r;
+}
+
+template <int NZ>
+__device__ __attribute__((always_inline)) inline void wrk_apply(const WC &c, const RkArgs &rk, int mode, int stage, int cc, const WF<NZ> &S,
+                                 const WOps<NZ> &o, const WRk &r, const ZTabs &tb) {
+    if (!c.store) return;
+    constexpr int NS = WDim<NZ>::S;
+    const int l = c.lane;
+    const double dt = r.dt;
+    if (mode == ZT_CN2) {
         const double dt2 = 0.5 * dt;
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-#pragma unroll
-            for (int s = 0; s <= NS; ++s) {
-                const int z = s < NS ? l + 64 * s : NZ;
-                if (s == NS && l != 0) continue;
-                const double t = s < NS ? S[cc].v[s % NS] : S[cc].top;
-                double fz = f0;
-                if (z > 0 && z < NZ && !c.q00) fz = tb.hou_li ? f0 * exp(tb.filz[z]) : f0 * tb.filz[z];
-                const size_t i = c.col + z;
-                double vs;
-                if (stage == 1) {
-                    vs = rk.q[cc][i] + dt2 * t;
-                    rk.qdf[cc][i] = vs;
-                } else {
-                    vs = rk.qdf[cc][i];
-                }
-                rk.q[cc][i] = vd * (fz * (vs + dt2 * t));
+        for (int s = 0; s <= NS; ++s) {
+            const int z = s < NS ? l + 64 * s : NZ;
+            if (s == NS && l != 0) continue;
+            const double t = s < NS ? S.v[s % NS] : S.top;
+            const double a0 = s < NS ? o.a.v[s % NS] : o.a.top;
+            double fz = r.f0;
+            if (z > 0 && z < NZ && !c.q00) fz = tb.hou_li ? r.f0 * exp(tb.filz[z]) : r.f0 * tb.filz[z];
+            const size_t i = c.col + z;
+            double vs;
+            if (stage == 1) {
+                vs = a0 + dt2 * t;
+                rk.qdf[cc][i] = vs;
+            } else {
+                vs = a0;
             }
+            rk.q[cc][i] = r.vd * (fz * (vs + dt2 * t));
         }
         return;
     }
     const double dt2 = 0.5 * dt, dt3 = dt / 3.0, dt6 = dt / 6.0;
-    const double D = rk.vdiss ? rk.vdiss[c.colidx] : sc[SC_DFAC] * vhdis;
-    const double ep = exp(D);
-    const double em = 1.0 / ep;
-    const double epf = ep * f0;
+    const double em = r.em, epf = r.epf;
 #pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-#pragma unroll
-        for (int s = 0; s <= NS; ++s) {
-            if (s == NS && l != 0) continue;
-            const int z = s < NS ? l + 64 * s : NZ;
-            const double t = s < NS ? S[cc].v[s % NS] : S[cc].top;
-            const size_t i = c.col + z;
-            if (stage == 1) {
-                const double sq = f0 * t;
-                const double q0 = rk.q[cc][i];
-                rk.qdi[cc][i] = q0;
-                rk.q[cc][i] = (q0 + dt2 * sq) * em;
-                rk.qdf[cc][i] = q0 + dt6 * sq;
-            } else if (stage == 2) {
-                const double sq = epf * t;
-                rk.q[cc][i] = em * (rk.qdi[cc][i] + dt2 * sq);
-                rk.qdf[cc][i] = rk.qdf[cc][i] + dt3 * sq;
-            } else if (stage == 3) {
-                const double sq = epf * t;
-                rk.q[cc][i] = (em * em) * (rk.qdi[cc][i] + dt * sq);
-                rk.qdf[cc][i] = rk.qdf[cc][i] + dt3 * sq;
-            } else {
-                const double sq = (epf * epf) * t;
-                rk.q[cc][i] = (em * em) * (rk.qdf[cc][i] + dt6 * sq);
-            }
+    for (int s = 0; s <= NS; ++s) {
+        if (s == NS && l != 0) continue;
+        const int z = s < NS ? l + 64 * s : NZ;
+        const double t = s < NS ? S.v[s % NS] : S.top;
+        const double a0 = s < NS ? o.a.v[s % NS] : o.a.top;
+        const size_t i = c.col + z;
+        if (stage == 2 || stage == 3) {
+            const double b0 = s < NS ? o.b.v[s % NS] : o.b.top;
+            const double sq = epf * t;
+            rk.q[cc][i] = (stage == 2 ? em : em * em) * (a0 + (stage == 2 ? dt2 : dt) * sq);
+            rk.qdf[cc][i] = b0 + dt3 * sq;
+        } else if (stage == 1) {
+            const double sq = r.f0 * t;
+            rk.qdi[cc][i] = a0;
+            rk.q[cc][i] = (a0 + dt2 * sq) * em;
+            rk.qdf[cc][i] = a0 + dt6 * sq;
+        } else {
+            const double sq = (epf * epf) * t;
+            rk.q[cc][i] = (em * em) * (a0 + dt6 * sq);
         }
     }
 }
@@ -719,6 +742,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
         WF<NZ> none;
         wxform<NZ, WK_DST, WK_NONE>(c, Pp, none, ws, tw, sc);
     }
+    // operands of the first component's update: in flight across the exchange
+    const bool upd = a.mode != ZT_STORE;
+    WOps<NZ> ops;
+    if (upd) wrk_load<NZ>(c, a.rk, a.mode, a.stage, 0, ops);
     // dxi/dt = r_y - q_z ; deta/dt = p_z - r_x ; dzeta/dt = q_x - p_y (341-367)
     xput(c, wsm, 0, R);
     xput(c, wsm, 1, Q);
@@ -751,12 +778,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
             St[cc].top += ex.top;
         }
     }
-    if (a.mode == ZT_STORE) {
+    if (!upd) {
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) wstore(c, a.svorts[cc], St[cc]);
         return;
     }
-    wrk_update<NZ>(c, a.rk, a.mode, a.stage, St, a.tb);
+    const WRk rr = wrk_setup<NZ>(c, a.rk, a.mode, a.tb);
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+        WOps<NZ> nxt;
+        if (cc < 2) wrk_load<NZ>(c, a.rk, a.mode, a.stage, cc + 1, nxt);
+        wrk_apply<NZ>(c, a.rk, a.mode, a.stage, cc, St[cc], ops, rr, a.tb);
+        if (cc < 2) ops = nxt;
+    }
 }
 
 // Single z operators on one volume (kernel checks of the wave path).
