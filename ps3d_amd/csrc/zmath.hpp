@@ -33,7 +33,11 @@ __device__ inline void phi_only(const Grid &g, double K, bool q00, int iz, doubl
     pp = div * (em - ef * ep);
 }
 
-__device__ inline Hyp hyp_all(const Grid &g, double K, double k2l2i, bool q00, int iz) {
+// The same functions from ep = exp(-K zp), em = exp(-K zm), ef = exp(-K lz) and
+// div = 1 / (1 - ef^2) supplied by the caller (the wave kernels build ep and em by
+// recurrence along z).
+__device__ inline Hyp hyp_from(const Grid &g, double K, double k2l2i, bool q00, int iz, double ep, double em,
+                               double ef, double div) {
     Hyp h;
     double zm, zp;
     zm_zp(g, iz, zm, zp);
@@ -44,11 +48,8 @@ __device__ inline Hyp hyp_all(const Grid &g, double K, double k2l2i, bool q00, i
         return h;
     }
     const double kl = K;
-    const double ef = exp(-kl * g.lz);
-    const double div = 1.0 / (1.0 - ef * ef);
     const double k2ifac = 0.5 * k2l2i;
     const double Lm = kl * zm, Lp = kl * zp;
-    const double ep = exp(-Lp), em = exp(-Lm);
     h.phim = div * (ep - ef * em);
     h.phip = div * (em - ef * ep);
     const double dphim = -kl * div * (ep + ef * em);
@@ -60,6 +61,14 @@ __device__ inline Hyp hyp_all(const Grid &g, double K, double k2l2i, bool q00, i
     h.dthetam = -k2ifac * ((Q * Lp - 1.0) * dphim - R * Lm * dphip);
     h.dthetap = -k2ifac * ((Q * Lm - 1.0) * dphip - R * Lp * dphim);
     return h;
+}
+
+__device__ inline Hyp hyp_all(const Grid &g, double K, double k2l2i, bool q00, int iz) {
+    if (q00) return hyp_from(g, K, k2l2i, true, iz, 0.0, 0.0, 0.0, 0.0);
+    double zm, zp;
+    zm_zp(g, iz, zm, zp);
+    const double ef = exp(-K * g.lz);
+    return hyp_from(g, K, k2l2i, false, iz, exp(-K * zp), exp(-K * zm), ef, 1.0 / (1.0 - ef * ef));
 }
 
 }  // namespace ps3d

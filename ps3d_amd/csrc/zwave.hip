@@ -163,6 +163,40 @@ __device__ inline void wmid(const WC &c, const WF<NZ> &F, WF<NZ> &M) {
     M.top = 0.5 * (rdlane(F.v[S - 1], 63) + F.top);
 }
 
+// exp(-K zp) and exp(-K zm) at the lane's levels z = lane + 64 s (and z = nz in
+// slot S), by recurrence along z: two exponentials per lane and one per column
+// instead of two per level.  The products add a few ulps to each value.
+template <int NZ>
+struct WExp {
+    double ep[WDim<NZ>::S + 1], em[WDim<NZ>::S + 1];
+    double ef, div;
+};
+template <int NZ>
+__device__ inline WExp<NZ> make_wexp(const WC &c, const Grid &g) {
+    constexpr int S = WDim<NZ>::S;
+    static_assert(NZ == 64 * S, "wave layout covers nz = 64 S");
+    WExp<NZ> x;
+    const double kd = c.K * g.dz;
+    const double e64 = exp(-64.0 * kd);
+    x.ef = exp(-c.K * g.lz);
+    x.div = 1.0 / (1.0 - x.ef * x.ef);
+    double e = exp(-kd * (double)c.lane);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        x.ep[s] = e;
+        e *= e64;
+    }
+    double f = exp(-kd * (double)(64 - c.lane));   // z = lane + 64 (S - 1): zm = dz (64 - lane)
+#pragma unroll
+    for (int s = S - 1; s >= 0; --s) {
+        x.em[s] = f;
+        f *= e64;
+    }
+    x.ep[S] = x.ef;
+    x.em[S] = 1.0;
+    return x;
+}
+
 // phi-(z), phi+(z) of the lane's interior levels (inversion_utils.f90:484-520)
 template <int NZ>
 struct WPhi {
@@ -171,8 +205,19 @@ struct WPhi {
 template <int NZ>
 __device__ inline WPhi<NZ> make_phi(const WC &c, const Grid &g) {
     WPhi<NZ> h;
+    const WExp<NZ> x = make_wexp<NZ>(c, g);
 #pragma unroll
-    for (int s = 0; s < WDim<NZ>::S; ++s) phi_only(g, c.K, c.q00, c.lane + 64 * s, h.pm[s], h.pp[s]);
+    for (int s = 0; s < WDim<NZ>::S; ++s) {
+        if (c.q00) {
+            double zm, zp;
+            zm_zp(g, c.lane + 64 * s, zm, zp);
+            h.pm[s] = zm / g.lz;
+            h.pp[s] = zp / g.lz;
+        } else {
+            h.pm[s] = x.div * (x.ep[s] - x.ef * x.em[s]);
+            h.pp[s] = x.div * (x.em[s] - x.ef * x.ep[s]);
+        }
+    }
     return h;
 }
 
@@ -598,10 +643,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     // bs (interior) and es (all z): boundary parts (94-104); green and d/dz of the
     // sine part (107-122); quad (0,0): the mean flow (150-165) in members 1, 2
     WF<NZ> Bs, As;
+    const WExp<NZ> wx = make_wexp<NZ>(c, g);
 #pragma unroll
     for (int s = 0; s <= S; ++s) {
         const int z = s < S ? l + 64 * s : NZ;
-        const Hyp h = hyp_all(g, c.K, c.k2l2i, c.q00, z);
+        const Hyp h = hyp_from(g, c.K, c.k2l2i, c.q00, z, wx.ep[s], wx.em[s], wx.ef, wx.div);
         const double rkz = a.rkz_sc * (double)z;
         const double green = -1.0 / (c.k2l2 + rkz * rkz);
         const double bs = d0 * h.thetam + dn * h.thetap;
