@@ -53,14 +53,25 @@ __device__ inline void fix_tile(Tile &t, int n, int zc) {
     t.zce = rem < zc ? rem : zc;
 }
 
+// Tile handled by this block.  Workgroups go to the 8 XCDs round-robin (XCD =
+// blockIdx.x mod 8); the remap hands each XCD a contiguous range of tiles, so the z
+// tiles of one line set -- which share cache lines (a tile of ZC < 16 levels covers
+// part of a 128-byte line) and pages -- meet in one L2.
+__device__ inline int tile_index() {
+    const int B = gridDim.x, i = blockIdx.x;
+    const int q = B >> 3, r = B & 7, x = i & 7, k = i >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 __device__ inline Tile make_tile(const LineArgs &a) {
     Tile t;
     t.n = a.n;
     t.ZC = a.ZC;
     t.nl = a.ZC / 2;
     t.LS = a.n + 1;
-    const int zt = blockIdx.x % a.nzt;
-    t.other = blockIdx.x / a.nzt;
+    const int ti = tile_index();
+    const int zt = ti % a.nzt;
+    t.other = ti / a.nzt;
     t.z0 = zt * a.ZC;
     const int rem = a.g.nz + 1 - t.z0;
     t.zce = rem < a.ZC ? rem : a.ZC;
@@ -561,7 +572,7 @@ k_line_reg(LineArgs a) {
         m1 = block_sum(m1, scratch);
         m2 = block_sum(m2, scratch);
         if (threadIdx.x == 0) {
-            double *p = a.part + (size_t)blockIdx.y * a.bt.part + (size_t)blockIdx.x * RY_N;
+            double *p = a.part + (size_t)blockIdx.y * a.bt.part + (size_t)tile_index() * RY_N;
             p[RY_UU] = suu;
             p[RY_WW] = sww;
             p[RY_UW] = suw;
@@ -787,7 +798,7 @@ k_line_reg(LineArgs a) {
         l1 = block_sum(l1, scratch);
         l2 = block_sum(l2, scratch);
         if (threadIdx.x == 0) {
-            double *p = a.part + (size_t)blockIdx.y * a.bt.part + (size_t)blockIdx.x * RX_N;
+            double *p = a.part + (size_t)blockIdx.y * a.bt.part + (size_t)tile_index() * RX_N;
             p[RX_W2MAX] = w2max;
             p[RX_GG] = gg;
             p[RX_USGG] = usgg;
