@@ -196,12 +196,14 @@ struct LineArgs {
     Batch bt;
     uint16_t *nn_in;   // LM_XMAIN: also write the closure input (bf16 [p][16]) when set
     float nn_scale[6];
-    int rthreads;      // register-holding modes: 1024 (standard tile) or 512 (half depth)
+    int rthreads;      // register-holding modes: 1024 (standard tile), 512 (half depth) or 256 (quarter)
 };
 
 size_t line_lds_bytes(int n, int ZC);
-// whether the 512-thread register build exists for line length n
-bool line_reg_half_ok(int n);
+// whether the T-thread register build (T = 256, 512, 1024) exists and holds a tile
+// of line length n, and the tile depth it takes
+bool line_reg_ok(int n, int T);
+int line_reg_zc(int n, int T);
 hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream);
 
 // Reductions: finishes the Parseval sums (after LM_INV_Y1) and the adapt step

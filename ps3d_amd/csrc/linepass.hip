@@ -25,10 +25,12 @@ __host__ __device__ constexpr int zc_for(int n) {
     return ((4096 / n > 16 ? 16 : (4096 / n < 2 ? 2 : 4096 / n)) & ~1);
 }
 // z depth of the register-holding modes at T threads per block: the 1024-thread
-// build takes the standard tile, the 512-thread build half of it (at least 2)
+// build takes the standard tile, T = 512 half of it and T = 256 a quarter (at least 2)
 __host__ __device__ constexpr int zc_reg(int n, int T) {
-    return T >= kLineThreads ? zc_for(n) : ((zc_for(n) / 2) < 2 ? 2 : ((zc_for(n) / 2) & ~1));
+    return T >= kLineThreads ? zc_for(n) : ((zc_for(n) * T / kLineThreads) < 2 ? 2 : ((zc_for(n) * T / kLineThreads) & ~1));
 }
+// the registers of T threads (kEpt complex values each) must hold the tile
+__host__ __device__ constexpr bool reg_fits(int n, int T) { return n * zc_reg(n, T) / 2 <= kEpt * T; }
 
 // Register-holding modes run 1024 threads per tile and use the radix-8 plans to
 // stay within 128 VGPRs; transform-only modes use the radix-16 plans.
@@ -477,7 +479,8 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
 // Modes that keep physical values in registers (products, adapt diagnostics); one
 // instantiation per mode M so that each gets its own register allocation.
 // T = 1024: one block per CU; T = 512 (half-depth tiles, at most 128 VGPRs): two
-// blocks per CU, so that one block's barriers overlap the other's loads.
+// blocks per CU, so that one block's barriers overlap the other's loads; T = 256:
+// four blocks per CU on quarter-depth tiles.
 template <int N, int M, int T>
 __global__ void __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= kLineThreads ? 1 : 4)))
 k_line_reg(LineArgs a) {
@@ -821,6 +824,8 @@ size_t line_lds_bytes(int n, int ZC) {
 }
 
 #define PS3D_LINE_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
+// lengths with a 256-thread (quarter-depth) register build
+#define PS3D_LINE_SIZES_Q(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
 
 hipError_t configure_linepass() {
     const int lds = 160 * 1024;
@@ -833,8 +838,11 @@ hipError_t configure_linepass() {
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_simple<N_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
     CFGT(N_, kLineThreads)
 #define CFG2(N_) CFG(N_) CFGT(N_, 512)
+#define CFG4(N_) CFGT(N_, 256)
     CFG(0)
     PS3D_LINE_SIZES(CFG2)
+    PS3D_LINE_SIZES_Q(CFG4)
+#undef CFG4
 #undef CFG2
 #undef CFG
 #undef CFGT
@@ -862,25 +870,36 @@ static void launch_line_t(const LineArgs &a, hipStream_t stream) {
         hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks, nb), dim3(256), lds, stream, a);
     else if (a.rthreads == 512 && N > 0)
         launch_reg_t<N, 512>(a, stream, blocks, nb, lds);
-    else
+    else if constexpr (N > 0 && N <= 512) {
+        if (a.rthreads == 256)
+            launch_reg_t<N, 256>(a, stream, blocks, nb, lds);
+        else
+            launch_reg_t<N, kLineThreads>(a, stream, blocks, nb, lds);
+    } else
         launch_reg_t<N, kLineThreads>(a, stream, blocks, nb, lds);
 }
 
-bool line_reg_half_ok(int n) {
+bool line_reg_ok(int n, int T) {
+    if (T == kLineThreads) return true;
+    bool built = false;
     switch (n) {
-#define CASE(N_) case N_: return true;
+#define CASE(N_) case N_: built = T == 512 || (T == 256 && N_ <= 512); break;
         PS3D_LINE_SIZES(CASE)
 #undef CASE
-        default: return false;
+        default: break;
     }
+    return built && reg_fits(n, T);
 }
+
+int line_reg_zc(int n, int T) { return zc_reg(n, T); }
 
 hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
     (void)nthreads;
     // compile-time plans need the standard tile depth of their build
     const bool reg = a.mode == LM_INV_Y1 || a.mode == LM_XMAIN || a.mode == LM_XADAPT;
-    const bool std_zc = a.ZC == (reg ? zc_reg(a.n, a.rthreads == 512 ? 512 : kLineThreads) : zc_for(a.n));
-    if (reg && a.rthreads == 512 && !std_zc) return hipErrorInvalidValue;
+    const int rt = (a.rthreads == 512 || a.rthreads == 256) ? a.rthreads : kLineThreads;
+    const bool std_zc = a.ZC == (reg ? zc_reg(a.n, rt) : zc_for(a.n));
+    if (reg && rt != kLineThreads && (!std_zc || !line_reg_ok(a.n, rt))) return hipErrorInvalidValue;
     switch (std_zc ? a.n : 0) {
 #define CASE(N_) case N_: launch_line_t<N_>(a, stream); break;
         PS3D_LINE_SIZES(CASE)

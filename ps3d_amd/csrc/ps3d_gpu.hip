@@ -156,14 +156,14 @@ static bool factor23(int n) {
     return n == 1;
 }
 
-static LineCfg line_cfg(int n, int nz, int zs, int nother, long estride, long ostride, bool half = false) {
+static LineCfg line_cfg(int n, int nz, int zs, int nother, long estride, long ostride, int rthreads = 0) {
     LineCfg c;
     c.n = n;
     int zc = 4096 / n;
     if (zc > 16) zc = 16;
     if (zc < 2) zc = 2;
     zc &= ~1;
-    if (half) zc = std::max(2, (zc / 2) & ~1);
+    if (rthreads) zc = ps3d::line_reg_zc(n, rthreads);
     c.ZC = zc;
     c.nzt = (nz + 1 + zc - 1) / zc;
     c.nother = nother;
@@ -325,20 +325,19 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     h->ly = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs);
     h->lx = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs);
     {
-        // 512-thread half-depth tiles (bit 0: x products pass, bit 1: adapt-stage y
-        // pass, bit 2: x adapt pass).  Measured at 256^3 all three gain; at 512^3 only
-        // the x adapt pass does.  PS3D_LINE512 overrides the choice.
-        const char *ev = std::getenv("PS3D_LINE512");
-        const int half = ev ? std::atoi(ev) : (std::max(nx, ny) <= 256 ? 7 : 4);
-        const bool hm = (half & 1) && ps3d::line_reg_half_ok(nx);
-        const bool hy = (half & 2) && ps3d::line_reg_half_ok(ny);
-        const bool hx = (half & 4) && ps3d::line_reg_half_ok(nx);
-        h->lxm = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, hm);
-        h->lyr = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs, hy);
-        h->lxr = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, hx);
-        h->rtm = hm ? 512 : 1024;
-        h->rty = hy ? 512 : 1024;
-        h->rtx = hx ? 512 : 1024;
+        // Block sizes of the register-holding line modes (x products pass, adapt-stage
+        // y pass, x adapt pass): 1024 threads on the standard tile, 512 on half of it,
+        // 256 on a quarter.  Measured fastest: 512, 512, 256 at 256^3 and 512 for all
+        // three at 512^3.  PS3D_LINE_T="m,y,x" overrides.
+        int tm = 512, ty = 512, tx = nx <= 256 ? 256 : 512;
+        if (const char *ev = std::getenv("PS3D_LINE_T")) std::sscanf(ev, "%d,%d,%d", &tm, &ty, &tx);
+        auto pick = [](int n, int T) { return (T != 1024 && ps3d::line_reg_ok(n, T)) ? T : 1024; };
+        h->rtm = pick(nx, tm);
+        h->rty = pick(ny, ty);
+        h->rtx = pick(nx, tx);
+        h->lxm = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, h->rtm);
+        h->lyr = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs, h->rty);
+        h->lxr = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, h->rtx);
     }
 
     HIPCHK(ps3d::configure_zpass());

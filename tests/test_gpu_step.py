@@ -248,23 +248,24 @@ def test_wave_and_quad_z_kernels_agree(G, monkeypatch, variant):
 
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (64, 48, 32), (256, 96, 16), (512, 32, 8)])
-@pytest.mark.parametrize("half", ["1", "2", "4", "7"])
-def test_half_depth_line_tiles_agree(oracle_mod, G, monkeypatch, shape, half):
-    """512-thread register line kernels on half-depth tiles (PS3D_LINE512 bit 0: x
-    products pass, bit 1: adapt-stage y pass, bit 2: x adapt pass) against the oracle
-    and the 1024-thread build, with adaptive dt so the adapt reductions steer the run."""
+@pytest.mark.parametrize("threads", ["512,1024,1024", "1024,512,1024", "1024,1024,512", "256,256,256"])
+def test_register_line_block_sizes_agree(oracle_mod, G, monkeypatch, shape, threads):
+    """Register line kernels at 512 and 256 threads per block on half- and
+    quarter-depth tiles (PS3D_LINE_T = x products, adapt-stage y, x adapt pass)
+    against the oracle and the 1024-thread build, with adaptive dt so the adapt
+    reductions steer the run."""
     nx, ny, nz = shape
     kw = dict(nx=nx, ny=ny, nz=nz, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI))
     cfg = config1(dt_fixed=0.0, pretype="roll-mean-max-strain", filtering="Hou & Li")
     vor = ics.beltrami_multimode(nx, ny, nz, seed=6)
-    monkeypatch.setenv("PS3D_LINE512", "0")
+    monkeypatch.setenv("PS3D_LINE_T", "1024,1024,1024")
     a = G.Ps3dGpu(G.make_params(**kw, **cfg))
-    monkeypatch.setenv("PS3D_LINE512", half)
+    monkeypatch.setenv("PS3D_LINE_T", threads)
     b = G.Ps3dGpu(G.make_params(**kw, **cfg))
     o = oracle_mod.Oracle(oracle_mod.make_params(**kw, **cfg))
     for m in (a, b, o):
         m.set_vorticity(vor)
-    print("half", half, shape, flush=True)
+    print("threads", threads, shape, flush=True)
     da, db, do = a.advance_n(3), b.advance_n(3), o.advance_n(3)
     assert field_rel(b.get(G.SVOR), a.get(G.SVOR)) < 1e-13
     assert field_rel(b.get(G.SVOR), o.get(oracle_mod.SVOR)) < TOL
