@@ -15,6 +15,7 @@ Prints one JSON object.
 import argparse
 import json
 import math
+import sys
 import time
 
 import numpy as np
@@ -24,6 +25,12 @@ from . import gpu as G
 from . import ics
 
 PI = math.pi
+
+
+def progress(msg):
+    """One progress line on stderr (the result is the JSON line on stdout)."""
+    sys.stderr.write("[closure_run] %s\n" % msg)
+    sys.stderr.flush()
 
 
 def params(n, limit=1.0e30):
@@ -54,6 +61,7 @@ def dataset(nf, nc, runs, snaps, every, spinup, seed0):
             T.append(t)
             f.advance_n(every)
         f.close()
+        progress("fine run %d/%d: %d snapshots" % (r + 1, runs, len(X)))
     coarse.close()
     tco.close()
     return X, T
@@ -79,6 +87,7 @@ def train(nc, X, T, epochs, lr, seed):
     for e in range(epochs):
         lr_e = lr * 0.5 * (1.0 + math.cos(math.pi * e / max(1, epochs)))
         hist.append(net.train_epoch(rng.permutation(len(X)), lr_e))
+        progress("epoch %d/%d: loss %.4f" % (e + 1, epochs, hist[-1]))
     return net, hist, time.perf_counter() - t0
 
 
@@ -88,14 +97,18 @@ def spectrum_of(h, t):
     return CL.energy_spectrum(h, row[G.DIAG_NAMES.index("ke")])
 
 
-def evaluate(nf, nc, net, seeds, t_end):
+def evaluate(nf, nc, net, seeds, t_end, gains=(1.0,)):
+    """Coarse runs without the closure and with it at each output gain (the net's
+    output scaled by g), against the truncated fine run at t_end."""
     out = []
     kcut = int(nc / 3)   # shells below the coarse 2/3 cutoff
+    base_scale = np.array(net.out_scale, dtype=np.float32)
     for s in seeds:
         k0 = 5.0
         f = fine_run(nf, k0, s, limit=t_end)
         runs = {}
-        for tag in ("no_closure", "closure"):
+        tags = ["no_closure"] + ["closure" if len(gains) == 1 else "gain_%g" % g for g in gains]
+        for tag in tags:
             c = G.Ps3dGpu(params(nc, limit=t_end))
             G.resample(c, f)                    # truncated initial state
             runs[tag] = c
@@ -109,8 +122,9 @@ def evaluate(nf, nc, net, seeds, t_end):
         G.resample(cref, f)
         _, ef = spectrum_of(cref, t_end)
         res = {"seed": s, "fine_steps": nfs, "fine_s": fine_s}
-        for tag, c in runs.items():
-            if tag == "closure":
+        for i, (tag, c) in enumerate(runs.items()):
+            if i > 0:
+                net.out_scale = base_scale * np.float32(gains[i - 1])
                 net.attach(c)
             n = 0
             t0 = time.perf_counter()
@@ -120,10 +134,12 @@ def evaluate(nf, nc, net, seeds, t_end):
             el = time.perf_counter() - t0
             _, ec = spectrum_of(c, t_end)
             res[tag] = {"steps": n, "s": el, "spectrum_error": CL.spectrum_error(ec, ef, kcut)}
-            if tag == "closure":
+            if i > 0:
                 CL.ClosureNet.detach(c)
+                net.out_scale = base_scale
             c.close()
         out.append(res)
+        progress("seed %d: %s" % (s, {k: round(v["spectrum_error"], 4) for k, v in res.items() if isinstance(v, dict)}))
         f.close()
         cref.close()
     return out
@@ -164,6 +180,9 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval-seeds", type=int, default=2)
     ap.add_argument("--t-end", type=float, default=0.5)
+    ap.add_argument("--gains", default="1.0",
+                    help="output gains tried on the validation seeds (comma separated); the best is evaluated")
+    ap.add_argument("--val-seeds", type=int, default=1)
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     X, T = dataset(a.fine, a.coarse, a.runs, a.snaps, a.every, a.spinup, 500)
@@ -172,6 +191,15 @@ def main(argv=None):
     fwd_ms, train_ms = net.timing(20)
     npts = a.coarse * a.coarse * (a.coarse + 1)
     gflop = CL.GFLOP_FWD_PER_POINT * npts
+    gains = [float(g) for g in a.gains.split(",")]
+    val = None
+    gain = gains[0]
+    if len(gains) > 1:
+        # output gain chosen on validation seeds (never the evaluation seeds)
+        val = evaluate(a.fine, a.coarse, net, [8001 + i for i in range(a.val_seeds)], a.t_end, gains)
+        err = [np.mean([v["gain_%g" % g]["spectrum_error"] for v in val]) for g in gains]
+        gain = gains[int(np.argmin(err))]
+    net.out_scale = net.out_scale * np.float32(gain)
     ev = evaluate(a.fine, a.coarse, net, [9001 + i for i in range(a.eval_seeds)], a.t_end)
     st = step_timing(a.fine, a.coarse, net)
     out = {
@@ -181,6 +209,7 @@ def main(argv=None):
         "forward_mfma_frac": gflop / fwd_ms * 1e12 / 2.5e15,
         "train_step_ms": train_ms,
         "step_ms": st, "speedup_per_step": st["fine"] / st["coarse_closure"],
+        "gain": gain, "validation": val,
         "eval": ev,
     }
     tf = [e["fine_s"] for e in ev]

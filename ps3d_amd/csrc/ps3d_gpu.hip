@@ -50,6 +50,10 @@ struct ps3d_handle {
     ps3d_params p{};
     int device = 0;
     hipStream_t stream = nullptr;
+    // one captured step replayed by advance_n (rebuilt when a captured argument --
+    // closure, diagnostics buffer, state set-up -- changes)
+    hipGraph_t sgraph = nullptr;
+    hipGraphExec_t sexec = nullptr;
     Grid g{};
     size_t nvol = 0;               // doubles per padded volume
     double *pool = nullptr;
@@ -192,9 +196,17 @@ static ZTabs ztabs(const ps3d_handle *h) {
     return t;
 }
 
+static void drop_step_graph(ps3d_handle *h) {
+    if (h->sexec) hipGraphExecDestroy(h->sexec);
+    if (h->sgraph) hipGraphDestroy(h->sgraph);
+    h->sexec = nullptr;
+    h->sgraph = nullptr;
+}
+
 static void free_handle(ps3d_handle *h) {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
+    drop_step_graph(h);
     hipFree(h->pool);
     hipFree(h->pres);
     hipFree(h->delta);
@@ -591,7 +603,7 @@ static int x_adapt(ps3d_handle *h, int phys_in) {
 }
 
 // x pass (u x omega) + y forward: r, q, p left semi-spectral in dst[]
-static int products(ps3d_handle *h, int phys_in, double *const *dst) {
+static int products(ps3d_handle *h, int phys_in, double *const *dst, bool nn_export = false) {
     LineArgs x = line_args(h, 0, LM_XMAIN);
     x.phys_in = phys_in;
     for (int c = 0; c < 3; ++c) {
@@ -599,7 +611,7 @@ static int products(ps3d_handle *h, int phys_in, double *const *dst) {
         x.in[3 + c] = h->vor[c];
         x.out[c] = dst[c];
     }
-    if (h->cnn) {   // the x pass also writes the closure input
+    if (h->cnn && nn_export) {   // the x pass also writes the closure input
         x.nn_in = ps3d::cnn_input(h->cnn);
         for (int k = 0; k < 6; ++k) x.nn_scale[k] = h->cnn_in[k];
     }
@@ -631,18 +643,21 @@ static int fused_stage(ps3d_handle *h, int mode, int stage) {
         prof_end(h);
         if (st) return st;
     }
-    if ((st = products(h, 0, h->vel))) return st;
+    if ((st = products(h, 0, h->vel, stage == 1))) return st;
     ZTendArgs t = ztend_args(h, mode, stage, h->vel);
     if (h->cnn) {
-        // closure: net on (u, v, w, omega) of this stage -> physical tendency in grad,
-        // decomposed like a physical field into mid, added to the source in the z pass
-        HIPCHK(ps3d::cnn_apply(h->cnn, h->cnn_out, h->g.zs, h->grad[0], h->grad[1], h->grad[2], h->stream));
-        if ((st = line_fields(h, 1, LM_FWD, 3, h->grad, h->grad))) return st;
-        if ((st = line_fields(h, 0, LM_FWD, 3, h->grad, h->grad))) return st;
-        for (int c = 0; c < 3; ++c) {
-            HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DECOMPOSE, h->grad[c], h->mid[c]), h->stream));
-            t.extra[c] = h->mid[c];
+        // closure: net on (u, v, w, omega) at the step's first stage -> physical
+        // tendency in grad, decomposed like a physical field into mid and added to the
+        // source of every stage of the step (held fixed over the step, as the
+        // subgrid term is updated once per step; mid is free after the adapt stage)
+        if (stage == 1) {
+            HIPCHK(ps3d::cnn_apply(h->cnn, h->cnn_out, h->g.zs, h->grad[0], h->grad[1], h->grad[2], h->stream));
+            if ((st = line_fields(h, 1, LM_FWD, 3, h->grad, h->grad))) return st;
+            if ((st = line_fields(h, 0, LM_FWD, 3, h->grad, h->grad))) return st;
+            for (int c = 0; c < 3; ++c)
+                HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_DECOMPOSE, h->grad[c], h->mid[c]), h->stream));
         }
+        for (int c = 0; c < 3; ++c) t.extra[c] = h->mid[c];
     }
     prof_begin(h, PS3D_K_ZTEND);
     HIPCHK(launch_ztend(h->zl, t, h->stream));
@@ -691,6 +706,7 @@ static int write_scal(ps3d_handle *h, int idx, double v) {
 
 static int ensure_diag(ps3d_handle *h, int rows) {
     if (rows <= h->diag_cap) return 0;
+    drop_step_graph(h);
     hipFree(h->diag);
     h->diag = nullptr;
     HIPCHK(hipMalloc((void **)&h->diag, sizeof(double) * PS3D_NDIAG * (size_t)rows * h->nbatch));
@@ -797,6 +813,7 @@ static int finish_setup(ps3d_handle *h);
 int ps3d_gpu_set_vorticity(ps3d_handle *h, const double *vor) {
     if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !vor) return fail(PS3D_E_ARG, "null argument");
+    drop_step_graph(h);
     const size_t hv = (size_t)h->g.nx * h->g.ny * (h->g.nz + 1);
     int st;
     for (int c = 0; c < 3; ++c)
@@ -926,9 +943,34 @@ int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_serie
     double zero = 0.0;
     HIPCHK(hipMemcpyAsync(h->scal + SC_T, t, sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->scal + SC_STEP, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    // from two steps on, the step is captured once and replayed (it has no host
+    // work: dt and the diagnostics row counter live on the device); PS3D_GRAPH=0
+    // launches every kernel instead
+    const char *gv = std::getenv("PS3D_GRAPH");
+    const bool graph = nsteps >= 2 && !(gv && gv[0] == '0');
+    if (graph && !h->sexec) {
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+        st = one_step(h);
+        hipGraph_t gr = nullptr;
+        const hipError_t ce = hipStreamEndCapture(h->stream, &gr);
+        if (st || ce != hipSuccess) {
+            if (gr) hipGraphDestroy(gr);
+            return st ? st : fail(PS3D_E_HIP, std::string("step capture: ") + hipGetErrorString(ce));
+        }
+        h->sgraph = gr;
+        const hipError_t ie = hipGraphInstantiate(&h->sexec, h->sgraph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            drop_step_graph(h);
+            return fail(PS3D_E_HIP, std::string("step graph: ") + hipGetErrorString(ie));
+        }
+    }
     HIPCHK(hipEventRecord(h->ev0, h->stream));
-    for (int s = 0; s < nsteps; ++s)
-        if ((st = one_step(h))) return st;
+    for (int s = 0; s < nsteps; ++s) {
+        if (graph)
+            HIPCHK(hipGraphLaunch(h->sexec, h->stream));
+        else if ((st = one_step(h)))
+            return st;
+    }
     HIPCHK(hipEventRecord(h->ev1, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     float ms = 0.0f;
@@ -1233,6 +1275,7 @@ int ps3d_gpu_resample(ps3d_handle *dst, ps3d_handle *src, int which) {
     if (dst->nbatch != 1 || src->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: resample one instance at a time");
     if (which != PS3D_SVOR && which != PS3D_SVORTS) return fail(PS3D_E_UNSUPPORTED, "resample: svor or svorts only");
     if (which == PS3D_SVOR && !src->have_state) return fail(PS3D_E_STATE, "resample of svor before set_vorticity");
+    drop_step_graph(dst);
     for (int d = 0; d < 3; ++d)
         if (dst->p.extent[d] != src->p.extent[d] || dst->p.lower[d] != src->p.lower[d])
             return fail(PS3D_E_ARG, "resample needs the same box");
@@ -1564,6 +1607,8 @@ int ps3d_gpu_ensemble_set_choice(ps3d_ensemble *e, int g, int code) {
 
 int ps3d_gpu_set_closure(ps3d_handle *h, ps3d_cnn *net, const float *in_scale, const float *out_scale) {
     if (!h) return fail(PS3D_E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    drop_step_graph(h);
     if (!net) {
         h->cnn = nullptr;
         return 0;

@@ -273,3 +273,28 @@ def test_register_line_block_sizes_agree(oracle_mod, G, monkeypatch, shape, thre
     assert max(r.values()) < TOL, r
     a.close()
     b.close()
+
+
+def test_step_graph_replays_equal_eager_launches(G, monkeypatch):
+    """advance_n replays one captured step; every replay, with eager work on the
+    same stream between the calls, equals the kernel-by-kernel launches exactly."""
+    p = G.make_params(**box(32), **config1(dt_fixed=0.0, filtering="Hou & Li",
+                                          pretype="roll-mean-max-strain"))
+    a, b = G.Ps3dGpu(p), G.Ps3dGpu(p)
+    vor = ics.beltrami_multimode(32, 32, 32, seed=8)
+    a.set_vorticity(vor)
+    b.set_vorticity(vor)
+    for rep in range(4):
+        print("replay round", rep, flush=True)
+        monkeypatch.setenv("PS3D_GRAPH", "0")
+        da = a.advance_n(3)
+        monkeypatch.delenv("PS3D_GRAPH")
+        db = b.advance_n(3)
+        assert np.array_equal(da, db)
+        assert np.array_equal(a.get(G.SVOR), b.get(G.SVOR))
+        for m in (a, b):      # eager work between the replays
+            m.vor2vel()
+            m.derived()
+        assert np.array_equal(a.get(G.PRES), b.get(G.PRES))
+    a.close()
+    b.close()
