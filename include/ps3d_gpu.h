@@ -158,6 +158,12 @@ int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_serie
  * PS3D_PRES / PS3D_DELTA. */
 int ps3d_gpu_derived(ps3d_handle *h);
 
+/* Hyperdiffusion coefficients: vvisc of init_diffusion (inversion_utils.f90:157-184)
+ * and the prefactor of the last adapt (get_diffusion_pre_factor, advance.f90:381-410);
+ * column (k, l) decays at the rate pref * visc * (k^2 + l^2)^nnu.  Either pointer may
+ * be NULL. */
+int ps3d_gpu_viscosity(ps3d_handle *h, double *visc, double *pref);
+
 /* Device <-> host copies of a field (enum ps3d_field). */
 int ps3d_gpu_get(ps3d_handle *h, int which, double *host);
 int ps3d_gpu_set(ps3d_handle *h, int which, const double *host);

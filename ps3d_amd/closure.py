@@ -158,22 +158,50 @@ def to_points(f):
     return np.ascontiguousarray(np.moveaxis(f, 0, -1).reshape(-1, f.shape[0]))
 
 
-def sample_from_fine(fine, coarse, tcoarse):
+def hyperdiffusion_rate(solver):
+    """Decay rate pref * visc * (k^2 + l^2)^nnu of every column (nx, ny) of solver
+    (impl_rk4.f90:37-53 integrating factors; visc and pref of its last set-up and
+    adapt)."""
+    visc, pref = solver.viscosity()
+    nx, ny = solver.nx, solver.ny
+    k = 2 * np.pi / solver.p.extent[0] * np.minimum(np.arange(nx), nx - np.arange(nx))
+    l = 2 * np.pi / solver.p.extent[1] * np.minimum(np.arange(ny), ny - np.arange(ny))
+    return pref * visc * (k[:, None] ** 2 + l[None, :] ** 2) ** solver.p.nnu
+
+
+def sample_from_fine(fine, coarse, tcoarse, diffusion=False):
     """One training pair at the current state of `fine` (its vor2vel and source
     run here): input = coarse (u, v, w, omega) of R(omega_f); target = R(S_f) -
-    S_c(R(omega_f)) in physical space on the coarse points."""
+    S_c(R(omega_f)) in physical space on the coarse points.  diffusion=True adds
+    the difference of the two grids' hyperdiffusion at the coarse wavenumbers,
+    (rate_c - rate_f) R(omega_f), so the target is the whole missing tendency."""
     fine.vor2vel()
+    if diffusion:
+        fine.adapt(fine.t)
     fine.source()
     G.resample(coarse, fine)                 # coarse state = truncated vorticity (set up, vor2vel)
     vel = coarse.get(G.VEL)
     vor = coarse.get(G.VOR)
+    if diffusion:
+        coarse.adapt(fine.t)
     coarse.source()
     sc = coarse.get(G.SVORTS)
     G.resample(tcoarse, fine, G.SVORTS)      # R(S_f)
     sf = tcoarse.get(G.SVORTS)
-    tau = np.stack([coarse.op(G.OP_COMBINE_PHYS, sf[c] - sc[c]) for c in range(3)])
+    d = sf - sc
+    if diffusion:
+        rate_f = hyperdiffusion_rate(fine)[np.ix_(_fold(fine.nx, coarse.nx), _fold(fine.ny, coarse.ny))]
+        rate = hyperdiffusion_rate(coarse) - rate_f
+        d = d + rate[None, :, :, None] * coarse.get(G.SVOR)
+    tau = np.stack([coarse.op(G.OP_COMBINE_PHYS, d[c]) for c in range(3)])
     x = to_points(np.concatenate([vel, vor]))
     return x.astype(np.float32), to_points(tau).astype(np.float32)
+
+
+def _fold(nf, nc):
+    """Fine-grid slot of each coarse slot with the same wavenumber magnitude."""
+    i = np.arange(nc)
+    return np.where(i <= nc // 2, i, nf - (nc - i))
 
 
 # --------------------------------------------------------------------------

@@ -48,7 +48,7 @@ def fine_run(nf, k0, seed, limit=1.0e30):
     return h
 
 
-def dataset(nf, nc, runs, snaps, every, spinup, seed0):
+def dataset(nf, nc, runs, snaps, every, spinup, seed0, diffusion=False):
     rng = np.random.default_rng(seed0)
     coarse, tco = G.Ps3dGpu(params(nc)), G.Ps3dGpu(params(nc))
     X, T = [], []
@@ -56,7 +56,7 @@ def dataset(nf, nc, runs, snaps, every, spinup, seed0):
         f = fine_run(nf, float(rng.uniform(3.0, 8.0)), seed0 + r)
         f.advance_n(spinup)
         for s in range(snaps):
-            x, t = CL.sample_from_fine(f, coarse, tco)
+            x, t = CL.sample_from_fine(f, coarse, tco, diffusion)
             X.append(x)
             T.append(t)
             f.advance_n(every)
@@ -128,12 +128,23 @@ def evaluate(nf, nc, net, seeds, t_end, gains=(1.0,)):
                 net.attach(c)
             n = 0
             t0 = time.perf_counter()
+            # a run whose step collapses (the closure destabilising it) is stopped at
+            # 20 times the fine run's step count and reported as failed
+            ok = True
             while c.t < t_end - 1e-12:
-                c.advance_n(10)
+                diag = c.advance_n(10)
                 n += 10
+                if not np.all(np.isfinite(diag)) or n > 20 * nfs:
+                    ok = False
+                    break
             el = time.perf_counter() - t0
-            _, ec = spectrum_of(c, t_end)
-            res[tag] = {"steps": n, "s": el, "spectrum_error": CL.spectrum_error(ec, ef, kcut)}
+            if ok:
+                _, ec = spectrum_of(c, t_end)
+                err = CL.spectrum_error(ec, ef, kcut)
+            else:
+                err = float("inf")
+            res[tag] = {"steps": n, "s": el, "spectrum_error": err, "completed": ok}
+            progress("seed %d %s: %d steps, error %.4f" % (s, tag, n, err))
             if i > 0:
                 CL.ClosureNet.detach(c)
                 net.out_scale = base_scale
@@ -183,9 +194,12 @@ def main(argv=None):
     ap.add_argument("--gains", default="1.0",
                     help="output gains tried on the validation seeds (comma separated); the best is evaluated")
     ap.add_argument("--val-seeds", type=int, default=1)
+    ap.add_argument("--target", choices=["transfer", "transfer+diffusion"], default="transfer",
+                    help="closure target: the subgrid nonlinear transfer, or also the difference of the "
+                         "two grids' hyperdiffusion")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
-    X, T = dataset(a.fine, a.coarse, a.runs, a.snaps, a.every, a.spinup, 500)
+    X, T = dataset(a.fine, a.coarse, a.runs, a.snaps, a.every, a.spinup, 500, a.target == "transfer+diffusion")
     t_data = time.perf_counter() - t0
     net, hist, t_train = train(a.coarse, X, T, a.epochs, a.lr, 0)
     fwd_ms, train_ms = net.timing(20)
@@ -203,7 +217,7 @@ def main(argv=None):
     ev = evaluate(a.fine, a.coarse, net, [9001 + i for i in range(a.eval_seeds)], a.t_end)
     st = step_timing(a.fine, a.coarse, net)
     out = {
-        "fine": a.fine, "coarse": a.coarse, "snapshots": len(X), "data_s": t_data,
+        "fine": a.fine, "coarse": a.coarse, "target": a.target, "snapshots": len(X), "data_s": t_data,
         "epochs": a.epochs, "train_s": t_train, "loss_first": hist[0], "loss_last": hist[-1],
         "forward_ms": fwd_ms, "forward_gflop": gflop, "forward_tflops": gflop / fwd_ms,
         "forward_mfma_frac": gflop / fwd_ms * 1e12 / 2.5e15,

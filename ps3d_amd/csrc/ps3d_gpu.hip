@@ -891,6 +891,17 @@ int ps3d_gpu_adapt(ps3d_handle *h, double t, double *dt, double *diag) {
     return 0;
 }
 
+int ps3d_gpu_viscosity(ps3d_handle *h, double *visc, double *pref) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
+    if (!h) return fail(PS3D_E_ARG, "null handle");
+    double sc[SC_NSCAL];
+    int st;
+    if ((st = read_scal(h, sc))) return st;
+    if (visc) *visc = sc[SC_VISC];
+    if (pref) *pref = sc[SC_PREF];
+    return 0;
+}
+
 int ps3d_gpu_source(ps3d_handle *h) {
     if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h) return fail(PS3D_E_ARG, "null handle");

@@ -64,6 +64,7 @@ def lib():
         L.ps3d_gpu_last_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.ps3d_gpu_profile_kernels.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ps3d_gpu_resample.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.ps3d_gpu_viscosity.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ps3d_gpu_create_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.ps3d_gpu_batch_size.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.ps3d_gpu_set_vorticity_batch.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -154,6 +155,13 @@ class Ps3dGpu:
 
     def source(self):
         _check(lib().ps3d_gpu_source(self.h))
+
+    def viscosity(self):
+        """(visc, prefactor of the last adapt): column (k, l) decays at the rate
+        pref * visc * (k^2 + l^2)^nnu."""
+        v, p = C.c_double(), C.c_double()
+        _check(lib().ps3d_gpu_viscosity(self.h, C.byref(v), C.byref(p)))
+        return v.value, p.value
 
     def adapt(self, t):
         dt = C.c_double()
