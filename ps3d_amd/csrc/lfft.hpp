@@ -101,10 +101,26 @@ PS3D_CPLAN8(1536, 8, 8, 8, 3)
 PS3D_CPLAN8(2048, 8, 8, 8, 4)
 #undef PS3D_CPLAN8
 
-// One Stockham pass with compile-time length, radix and span.
-template <int N, int R, int P>
+// Element x of the complex line whose inverse FFT gives two real lines from their
+// Hermitian real spectra (stafft.f90:55-58): slot k < n/2 holds the real parts
+// (a(k), b(k)) and slot n-k the imaginary parts; scaled by hs.
+template <int N>
+__host__ __device__ inline double2 herm_in(const double2 *s, int x, double hs) {
+    constexpr int H = N / 2;
+    if (x == 0 || x == H) {
+        const double2 v = s[x];
+        return make_double2(hs * v.x, hs * v.y);
+    }
+    const double2 a = s[x < H ? x : N - x], b = s[x < H ? N - x : x];   // real, imaginary slots
+    return x < H ? make_double2(hs * (a.x - b.y), hs * (b.x + a.y)) : make_double2(hs * (a.x + b.y), hs * (a.y - b.x));
+}
+
+// One Stockham pass with compile-time length, radix and span.  HERM (first pass
+// only): the input lines are Hermitian real spectra, combined on the fly.
+template <int N, int R, int P, bool HERM = false>
 __host__ __device__ inline void cpass(const double2 *src, double2 *dst, int nlines, int ls, const double2 *tw, int dir,
-                                      int e0, int de) {
+                                      int e0, int de, double hs = 1.0) {
+    static_assert(!HERM || P == 1, "Hermitian input only in the first pass");
     constexpr int m = N / R, tstep = N / (P * R);
     const int tot = nlines * m;
     for (int e = e0; e < tot; e += de) {
@@ -116,7 +132,7 @@ __host__ __device__ inline void cpass(const double2 *src, double2 *dst, int nlin
         double2 u[R];
 #pragma unroll
         for (int t = 0; t < R; ++t) {
-            double2 v = s[i + t * m];
+            double2 v = HERM ? herm_in<N>(s, i + t * m, hs) : s[i + t * m];
             if (P > 1 && t > 0) {
                 const double2 w = tw[k * t * tstep];
                 v = dir < 0 ? cmul(v, w) : cmulc(v, w);
@@ -133,12 +149,14 @@ __host__ __device__ inline void cpass(const double2 *src, double2 *dst, int nlin
 #ifdef __HIPCC__
 // Batched complex FFT of `nlines` lines of length N in LDS (stride ls), ping-pong
 // with b; tw[m] = exp(-2 pi i m/N).  All threads of the block call it.
-template <int N, typename C = CPlan<N>>
+// HERM: the input lines hold Hermitian real spectra (scaled by hs on the way in),
+// which saves the separate pass that would form the complex lines.
+template <int N, typename C = CPlan<N>, bool HERM = false>
 __device__ inline double2 *cfft_lds(double2 *a, double2 *b, int nlines, int ls, const double2 *__restrict__ tw,
-                                    int dir) {
+                                    int dir, double hs = 1.0) {
     static_assert(C::ok, "no compile-time plan for this length");
     const int tid = threadIdx.x, nthr = blockDim.x;
-    cpass<N, C::r0, 1>(a, b, nlines, ls, tw, dir, tid, nthr);
+    cpass<N, C::r0, 1, HERM>(a, b, nlines, ls, tw, dir, tid, nthr, hs);
     __syncthreads();
     if constexpr (C::r1 == 0) {
         return b;

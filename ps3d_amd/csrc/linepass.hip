@@ -103,11 +103,18 @@ __device__ inline void tile_store(const Tile &t, const LineArgs &a, double *__re
 }
 
 // Hermitian pair -> complex spectrum, then inverse FFT.  Returns result buffer,
-// values to be scaled by 1/n.
+// values to be scaled by 1/n.  With a compile-time plan the pair is formed inside
+// the first FFT pass.
 template <int N, bool REG = false>
 __device__ inline double2 *tile_inverse(const Tile &t, const LineArgs &a, double2 *A, double2 *B) {
     const int n = t.n, h = n / 2;
     const double sn = sqrt((double)n);
+    if constexpr (N > 0) {
+        if constexpr (REG && CPlan8<N>::ok)
+            return cfft_lds<N, CPlan8<N>, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
+        else
+            return cfft_lds<N, CPlan<N>, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
+    }
     const int tot = t.nl * (h + 1);
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int line = e / (h + 1), k = e % (h + 1);

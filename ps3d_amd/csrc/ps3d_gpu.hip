@@ -339,9 +339,10 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     {
         // Block sizes of the register-holding line modes (x products pass, adapt-stage
         // y pass, x adapt pass): 1024 threads on the standard tile, 512 on half of it,
-        // 256 on a quarter.  Measured fastest: 512, 512, 256 at 256^3 and 512 for all
-        // three at 512^3.  PS3D_LINE_T="m,y,x" overrides.
-        int tm = 512, ty = 512, tx = nx <= 256 ? 256 : 512;
+        // 256 on a quarter.  Measured fastest: 512, 512, 256 at 256^3, 512 for all
+        // three at 512^3, and 1024, 1024, 512 at 1024^3 (where a half tile holds two z
+        // levels).  PS3D_LINE_T="m,y,x" overrides.
+        int tm = nx <= 512 ? 512 : 1024, ty = ny <= 512 ? 512 : 1024, tx = nx <= 256 ? 256 : 512;
         if (const char *ev = std::getenv("PS3D_LINE_T")) std::sscanf(ev, "%d,%d,%d", &tm, &ty, &tx);
         auto pick = [](int n, int T) { return (T != 1024 && ps3d::line_reg_ok(n, T)) ? T : 1024; };
         h->rtm = pick(nx, tm);
