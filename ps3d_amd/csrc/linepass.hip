@@ -153,6 +153,36 @@ __device__ inline double2 *tile_forward(const Tile &t, const LineArgs &a, double
     return R;
 }
 
+// Store of a forward-transformed tile (complex FFT output R, unscaled) as the
+// Hermitian real spectra of its two z levels per line, split on the fly: slot
+// k <= n/2 takes (Re A_k, Re B_k), slot n-k (Im A_k, Im B_k), scaled by 1/sqrt(n)
+// (the split of tile_forward without its LDS pass).  Each thread writes both
+// levels of one slot (16 contiguous bytes).
+template <int N>
+__device__ inline void tile_store_fwd(const Tile &t, const LineArgs &a, double *__restrict__ dst, const double2 *R) {
+    constexpr int H = N / 2;
+    const double sc = 0.5 / sqrt((double)N);
+    const int tot = t.nl * N;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+        const int p = e % t.nl, j = e / t.nl;
+        const double2 *L = R + (size_t)p * t.LS;
+        const double2 zj = L[j], zm = L[j == 0 ? 0 : N - j];
+        double2 v;
+        if (j <= H)
+            v = make_double2(sc * (zj.x + zm.x), sc * (zj.y + zm.y));
+        else
+            v = make_double2(sc * (zm.y - zj.y), sc * (zj.x - zm.x));
+        const int z = 2 * p;
+        double *o = dst + t.base + (long)j * a.estride + z;
+        if (z + 1 < t.zce) {
+            o[0] = v.x;
+            o[1] = v.y;
+        } else if (z < t.zce) {
+            o[0] = v.x;
+        }
+    }
+}
+
 // Spectral derivative along the line (diffx / diffy, sta3dfft.f90:304-377):
 // dst[j] = si(j) rk(j) src[n-j], zero at j = 0 and j = n/2.
 __device__ inline void tile_diff(const Tile &t, const LineArgs &a, const double2 *src, double2 *dst) {
@@ -241,8 +271,13 @@ __device__ inline void fwd_store(const Tile &t, const LineArgs &a, double2 *A, d
                                  double *dst) {
     from_regs(t, A, r);
     __syncthreads();
-    double2 *R = tile_forward<N, true>(t, a, A, B);
-    tile_store(t, a, dst, R, 1.0);
+    if constexpr (N > 0) {
+        double2 *R = lfft<N, true>(t, a, A, B, -1);
+        tile_store_fwd<N>(t, a, dst, R);
+    } else {
+        double2 *R = tile_forward<N, true>(t, a, A, B);
+        tile_store(t, a, dst, R, 1.0);
+    }
     __syncthreads();
 }
 
@@ -472,8 +507,8 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
             __syncthreads();
             if (f + 1 < a.nf) gload(a.in[f + 1]);
             if (a.mode == LM_FWD) {
-                double2 *R = tile_forward<N>(t, a, A, B);
-                tile_store(t, a, a.out[f], R, 1.0);
+                double2 *R = lfft<N>(t, a, A, B, -1);
+                tile_store_fwd<N>(t, a, a.out[f], R);
             } else {
                 double2 *R = tile_inverse<N>(t, a, A, B);
                 tile_store(t, a, a.out[f], R, 1.0 / (double)t.n);
@@ -675,9 +710,9 @@ k_line_reg(LineArgs a) {
             from_regs(t, A, r);
             from_regs(t, A + tb, q);
             __syncthreads();
-            double2 *R = tile_forward<N, true>(t2, a, A, B);
-            tile_store(t, a, a.out[0], R, 1.0);
-            tile_store(t, a, a.out[1], R + tb, 1.0);
+            double2 *R = lfft<N, true>(t2, a, A, B, -1);
+            tile_store_fwd<N>(t, a, a.out[0], R);
+            tile_store_fwd<N>(t, a, a.out[1], R + tb);
             __syncthreads();
         } else {
             fwd_store<N>(t, a, A, B, r, a.out[0]);
