@@ -82,18 +82,46 @@ __device__ inline Tile make_tile(const LineArgs &a) {
     return t;
 }
 
+// Global accesses move both z levels of a line slot at once (16 bytes): every
+// tile start, column stride and volume offset is even, so the pair is aligned.
+// A tile whose last level is odd reads one level past it (inside the column's
+// padding) and zeroes it.
+__device__ inline double2 gpair(const Tile &t, const LineArgs &a, const double *__restrict__ src, int j, int p) {
+    const int z = 2 * p;
+    if (z >= t.zce) return make_double2(0.0, 0.0);
+    const double2 v = *reinterpret_cast<const double2 *>(src + t.base + (long)j * a.estride + z);
+    return z + 1 < t.zce ? v : make_double2(v.x, 0.0);
+}
+__device__ inline void spair(const Tile &t, const LineArgs &a, double *__restrict__ dst, int j, int p, double2 v) {
+    const int z = 2 * p;
+    double *o = dst + t.base + (long)j * a.estride + z;
+    if (z + 1 < t.zce)
+        *reinterpret_cast<double2 *>(o) = v;
+    else if (z < t.zce)
+        o[0] = v.x;
+}
+
 __device__ inline void tile_load(const Tile &t, const LineArgs &a, const double *__restrict__ src, double2 *buf) {
-    double *d = reinterpret_cast<double *>(buf);
-    const int tot = t.n * t.ZC;
+    const int tot = t.n * t.nl;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-        const int zz = e % t.ZC, j = e / t.ZC;
-        const double v = zz < t.zce ? src[t.base + (long)j * a.estride + zz] : 0.0;
-        d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = v;
+        const int p = e % t.nl, j = e / t.nl;
+        buf[(size_t)p * t.LS + j] = gpair(t, a, src, j, p);
     }
 }
 
+// Deep tiles (16 levels: 128-byte runs) store level pairs; shallower tiles store
+// single levels, so that a wave's stores span fewer line slots.
 __device__ inline void tile_store(const Tile &t, const LineArgs &a, double *__restrict__ dst, const double2 *buf,
                                   double scale) {
+    if (t.ZC >= 16) {
+        const int tot = t.n * t.nl;
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            const int p = e % t.nl, j = e / t.nl;
+            const double2 v = buf[(size_t)p * t.LS + j];
+            spair(t, a, dst, j, p, make_double2(v.x * scale, v.y * scale));
+        }
+        return;
+    }
     const double *d = reinterpret_cast<const double *>(buf);
     const int tot = t.n * t.ZC;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -172,14 +200,7 @@ __device__ inline void tile_store_fwd(const Tile &t, const LineArgs &a, double *
             v = make_double2(sc * (zj.x + zm.x), sc * (zj.y + zm.y));
         else
             v = make_double2(sc * (zm.y - zj.y), sc * (zj.x - zm.x));
-        const int z = 2 * p;
-        double *o = dst + t.base + (long)j * a.estride + z;
-        if (z + 1 < t.zce) {
-            o[0] = v.x;
-            o[1] = v.y;
-        } else if (z < t.zce) {
-            o[0] = v.x;
-        }
+        spair(t, a, dst, j, p, v);
     }
 }
 
@@ -301,6 +322,8 @@ __device__ inline void nn_export(const Tile &t, const LineArgs &a, const double2
 
 // Register prefetch of a tile for the register-holding kernels (T threads): the
 // global loads of the next field are issued before the transform of the current one.
+// (8 bytes per thread: a wave's load then spans half as many line slots as with
+// the 16-byte pairs, which the x passes -- slots a whole y-z plane apart -- prefer)
 template <int N, int T = kLineThreads>
 struct Pf {
     static constexpr int ZC = N > 0 ? zc_reg(N, T) : 2;
@@ -487,22 +510,37 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
         return;
     }
     if constexpr (N > 0) {
-        constexpr int ZC = zc_for(N), PER = N * ZC / 256;
-        double pre[PER];
+        // deep tiles load level pairs (16 bytes), shallower ones single levels
+        constexpr int ZC = zc_for(N), NL = ZC / 2;
+        constexpr bool PAIR = ZC >= 16;
+        constexpr int PER = PAIR ? (N * NL + 255) / 256 : (N * ZC + 255) / 256;
+        double2 pre[PAIR ? PER : 1];
+        double pre1[PAIR ? 1 : PER];
         auto gload = [&](const double *__restrict__ src) {
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
-                const int e = threadIdx.x + k * 256, zz = e % ZC, j = e / ZC;
-                pre[k] = zz < t.zce ? src[t.base + (long)j * a.estride + zz] : 0.0;
+                const int e = threadIdx.x + k * 256;
+                if constexpr (PAIR) {
+                    const int q = e % NL, j = e / NL;
+                    pre[k] = j < N ? gpair(t, a, src, j, q) : make_double2(0.0, 0.0);
+                } else {
+                    const int zz = e % ZC, j = e / ZC;
+                    pre1[k] = (j < N && zz < t.zce) ? src[t.base + (long)j * a.estride + zz] : 0.0;
+                }
             }
         };
         gload(a.in[0]);
         for (int f = 0; f < a.nf; ++f) {
-            double *d = reinterpret_cast<double *>(A);
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
-                const int e = threadIdx.x + k * 256, zz = e % ZC, j = e / ZC;
-                d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = pre[k];
+                const int e = threadIdx.x + k * 256;
+                if constexpr (PAIR) {
+                    const int q = e % NL, j = e / NL;
+                    if (j < N) A[(size_t)q * t.LS + j] = pre[k];
+                } else {
+                    const int zz = e % ZC, j = e / ZC;
+                    if (j < N) reinterpret_cast<double *>(A)[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = pre1[k];
+                }
             }
             __syncthreads();
             if (f + 1 < a.nf) gload(a.in[f + 1]);
