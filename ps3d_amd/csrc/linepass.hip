@@ -131,13 +131,15 @@ __device__ inline void tile_store(const Tile &t, const LineArgs &a, double *__re
 }
 
 // Hermitian pair -> complex spectrum, then inverse FFT.  Returns result buffer,
-// values to be scaled by 1/n.  With a compile-time plan the pair is formed inside
-// the first FFT pass.
+// values to be scaled by 1/n.  With a compile-time plan of length >= 128 the pair
+// is formed inside the first FFT pass.
 template <int N, bool REG = false>
 __device__ inline double2 *tile_inverse(const Tile &t, const LineArgs &a, double2 *A, double2 *B) {
     const int n = t.n, h = n / 2;
     const double sn = sqrt((double)n);
-    if constexpr (N > 0) {
+    // (measured: the fold pays from n = 128 up; the small grids of an ensemble run
+    // faster with the separate pass)
+    if constexpr (N >= 128) {
         if constexpr (REG && CPlan8<N>::ok)
             return cfft_lds<N, CPlan8<N>, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
         else
