@@ -1,6 +1,6 @@
 """Learned subgrid closure for a coarse grid: the 3x3x3 convolution net of
 ps3d_closure.h (bf16 matrix cores), its training data from filtered fine-grid
-runs, training, and the genspec-style energy spectrum used to judge it.
+runs, training, and the genspec-style energy spectrum used to evaluate it.
 
 Training target: tau = R(S_f) - S_c(R(omega_f)), where S is ps3d's vorticity
 tendency (src/inversion/inversion.f90:298-388), R the sharp spectral truncation
