@@ -994,8 +994,11 @@ hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
 // ---------------------------------------------------------------------------
 // finishing reductions (one block)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_fin_y(FinArgs a) {
-    __shared__ double red[256];
+// one block per instance; 1024 threads read the per-tile partials
+constexpr int kFinThreads = 1024;
+
+__global__ void __launch_bounds__(kFinThreads) k_fin_y(FinArgs a) {
+    __shared__ double red[kFinThreads];
     const double *party = a.party + (size_t)blockIdx.y * a.bt.party;
     double *scal = a.scal + (size_t)blockIdx.y * SC_NSCAL;
     double s[RY_N];
@@ -1017,8 +1020,8 @@ __global__ void __launch_bounds__(256) k_fin_y(FinArgs a) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_fin_x(FinArgs a) {
-    __shared__ double red[256];
+__global__ void __launch_bounds__(kFinThreads) k_fin_x(FinArgs a) {
+    __shared__ double red[kFinThreads];
     double w2 = 0.0, gg = 2.220446049250313e-16, us = 0.0, ls = 0.0;
     double um = -INFINITY, vm = -INFINITY, wm = -INFINITY, l1 = 0.0, l2 = 0.0;
     const double *partx = a.partx + (size_t)blockIdx.y * a.bt.partx;
@@ -1108,11 +1111,11 @@ __global__ void __launch_bounds__(256) k_fin_x(FinArgs a) {
 }
 
 hipError_t launch_fin_y(const FinArgs &a, hipStream_t stream) {
-    hipLaunchKernelGGL(k_fin_y, dim3(1, a.bt.nb > 1 ? a.bt.nb : 1), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_fin_y, dim3(1, a.bt.nb > 1 ? a.bt.nb : 1), dim3(kFinThreads), 0, stream, a);
     return hipGetLastError();
 }
 hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream) {
-    hipLaunchKernelGGL(k_fin_x, dim3(1, a.bt.nb > 1 ? a.bt.nb : 1), dim3(256), 0, stream, a);
+    hipLaunchKernelGGL(k_fin_x, dim3(1, a.bt.nb > 1 ? a.bt.nb : 1), dim3(kFinThreads), 0, stream, a);
     return hipGetLastError();
 }
 
