@@ -41,6 +41,11 @@ int ps3d_nc_append_fields(const char *path, double t, int nx, int ny, int nz, co
 int ps3d_nc_read_fields(const char *path, int step, double *t, int nx, int ny, int nz, double *vel,
                         double *vor);
 
+/* = get_netcdf_box (src/netcdf/netcdf_reader.f90:319-337), as setup_domain_and_parameters
+ * uses it (src/utils/utils.f90:91-134): the grid and box of a field file from its
+ * global attributes ncells, extent and origin. */
+int ps3d_nc_grid(const char *path, int *ncells, double *extent, double *origin);
+
 /* Number of records of a field or field_stats file and the time of the last one
  * (t_last may be NULL; it is 0 when there are no records): what ps3d reads to
  * resume an existing file, with get_num_steps and get_time

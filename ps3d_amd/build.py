@@ -1,4 +1,5 @@
-"""Builds the HIP library libps3d_gpu.so (gfx950) in-tree with hipcc.
+"""Builds the HIP library libps3d_gpu.so (gfx950) in-tree with hipcc, and the
+Fortran host program ps3d_driver on top of it with amdflang.
 
 Usage: python -m ps3d_amd.build [--verbose]
 """
@@ -40,6 +41,36 @@ def _compile(src, verbose=False, extra=()):
     return obj
 
 
+FC = os.environ.get("PS3D_FC", "/opt/rocm/bin/amdflang")
+FSRC = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(LIBDIR, "ps3d_driver")
+
+
+def build_driver(verbose=False):
+    """The Fortran host program (ps3d_amd/fortran): the iso_c_binding module and
+    the ps3d driver, linked against libps3d_gpu.so found next to it."""
+    fb = os.path.join(BUILD, "fortran")
+    os.makedirs(fb, exist_ok=True)
+    objs = []
+    for src in ("ps3d_gpu.f90", "ps3d_driver.f90"):
+        obj = os.path.join(fb, os.path.splitext(src)[0] + ".o")
+        cmd = [FC, "-O2", "-module-dir", fb, "-c", os.path.join(FSRC, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("%s failed for %s:\n%s\n%s" % (FC, src, " ".join(cmd), r.stderr[-4000:]))
+        if verbose and r.stderr.strip():
+            sys.stderr.write(r.stderr)
+        objs.append(obj)
+    tmp = DRIVER + ".tmp"
+    cmd = [FC, "-o", tmp] + objs + ["-L" + LIBDIR, "-lps3d_gpu", "-Wl,-rpath,$ORIGIN",
+                                    "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 def build(verbose=False, extra=()):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -51,6 +82,7 @@ def build(verbose=False, extra=()):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
     os.replace(tmp, LIB)
+    build_driver(verbose)
     return LIB
 
 

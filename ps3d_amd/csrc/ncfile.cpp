@@ -533,6 +533,28 @@ int ps3d_nc_records(const char *path, int *nrecords, double *t_last) {
     return 0;
 }
 
+int ps3d_nc_grid(const char *path, int *ncells, double *extent, double *origin) {
+    if (!path || !ncells || !extent || !origin) return nfail("invalid argument");
+    Header h;
+    int st = read_header(path, h);
+    if (st) return st;
+    bool got[3] = {false, false, false};
+    for (const Att &a : h.gatts) {
+        if (a.name == "ncells" && a.type == NC_INT && a.ints.size() == 3) {
+            for (int k = 0; k < 3; ++k) ncells[k] = a.ints[k];
+            got[0] = true;
+        } else if (a.name == "extent" && a.type == NC_DOUBLE && a.dbls.size() == 3) {
+            for (int k = 0; k < 3; ++k) extent[k] = a.dbls[k];
+            got[1] = true;
+        } else if (a.name == "origin" && a.type == NC_DOUBLE && a.dbls.size() == 3) {
+            for (int k = 0; k < 3; ++k) origin[k] = a.dbls[k];
+            got[2] = true;
+        }
+    }
+    if (!got[0] || !got[1] || !got[2]) return nfail("file lacks the ncells / extent / origin attributes");
+    return 0;
+}
+
 int ps3d_nc_create_stats(const char *path) {
     if (!path) return nfail("invalid argument");
     Header h;

@@ -1,0 +1,116 @@
+"""The Fortran host program (ps3d_amd/fortran/ps3d_driver.f90): ps3d's &PS3D
+namelist config, field file input, time loop and NetCDF output on the library.
+
+CPU tests run the failure paths that stop before any GPU call; the GPU test runs a
+32^3 Beltrami config and compares every output record with the same run stepped
+from Python through the C ABI."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from ps3d_amd import build, gpu
+
+HAVE = os.path.exists(build.DRIVER) and os.path.exists(gpu.LIB_PATH)
+PI = np.pi
+
+CONFIG = """&PS3D
+ field_file              = '{field}'
+ vor_visc%nnu = 3
+ vor_visc%prediss = 30.0
+ vor_visc%pretype = 'vorch'
+ vor_visc%length_scale = 'Kolmogorov'
+ stepper = 'impl-diff-rk4'
+ filtering = '2/3-rule'
+ output%field_freq         = 0.02
+ output%field_stats_freq   = 0.01
+ output%write_fields       = .true.
+ output%overwrite          = .true.
+ output%basename           = '{base}'
+ time%limit              = 0.05
+ time%alpha              = 0.1
+ time%precise_stop       = .false.
+ time%dt_fixed           = 0.01
+/
+"""
+
+
+def run_driver(cfg, timeout=120):
+    return subprocess.run([build.DRIVER, "--config", cfg], capture_output=True, text=True, timeout=timeout)
+
+
+@pytest.mark.skipif(not HAVE, reason="driver not built")
+def test_driver_usage_and_missing_config(tmp_path):
+    r = subprocess.run([build.DRIVER], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "Usage" in r.stdout
+    r = run_driver(str(tmp_path / "none.config"))
+    assert r.returncode != 0 and "does not exist" in r.stdout
+
+
+@pytest.mark.skipif(not HAVE, reason="driver not built")
+def test_driver_rejects_bad_namelist_and_missing_field_file(tmp_path):
+    bad = tmp_path / "bad.config"
+    bad.write_text("&PS3D\n no_such_entry = 1\n/\n")
+    r = run_driver(str(bad))
+    assert r.returncode != 0 and "invalid Namelist" in r.stdout
+    cfg = tmp_path / "run.config"
+    cfg.write_text(CONFIG.format(field=str(tmp_path / "missing.nc"), base=str(tmp_path / "out")))
+    r = run_driver(str(cfg))
+    assert r.returncode != 0 and "field file grid" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not HAVE, reason="driver not built")
+def test_driver_run_matches_the_library_stepped_from_python(tmp_path):
+    from scipy.io import netcdf_file
+
+    from ps3d_amd import ics, ncio
+    n = 32
+    kw = dict(lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI))
+    vor0 = ics.beltrami_multimode(n, n, n, seed=1)
+    # initial field file, as ps3d's generator writes it (vorticity; velocity unused)
+    ic = ncio.FieldFile(str(tmp_path / "ic"), n, n, n, kw["lower"], kw["extent"], derived=False)
+    ic.write(0.0, np.zeros_like(vor0), vor0)
+    base = str(tmp_path / "beltrami_32")
+    cfg = tmp_path / "beltrami_32.config"
+    cfg.write_text(CONFIG.format(field=ic.path, base=base))
+    print("driver run", flush=True)
+    r = run_driver(str(cfg))
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    p = gpu.make_params(n, n, n, **kw, filtering="2/3-rule", stepper="impl-diff-rk4", nnu=3, prediss=30.0,
+                        pretype="vorch", length_scale="Kolmogorov", alpha=0.1, dt_fixed=0.01)
+    d = gpu.Ps3dGpu(p)
+    d.set_vorticity(vor0)
+    # the driver's loop (write_step before each step, forced write at the end)
+    eps = np.finfo(float).eps
+    ref, rows = [], []
+    t, nfw, nsfw = 0.0, 0, 0
+    while t < 0.05:
+        if t + eps >= nfw * 0.02:
+            d.vor2vel()
+            ref.append((t, d.get(gpu.VOR), d.get(gpu.VEL)))
+            nfw += 1
+        row = d.advance_n(1)[0]
+        t = d.t
+        if row[0] + eps >= nsfw * 0.01:
+            rows.append(row)
+            nsfw += 1
+    d.vor2vel()
+    ref.append((t, d.get(gpu.VOR), d.get(gpu.VEL)))
+    with netcdf_file(base + "_fields.nc", "r", mmap=False) as f:
+        times = np.array(f.variables["t"].data)
+        np.testing.assert_array_equal(times, [r[0] for r in ref])
+        for k, (tk, vor, vel) in enumerate(ref):
+            for c, x in enumerate("xyz"):
+                np.testing.assert_allclose(f.variables[x + "_vorticity"][k], vor[c].T, rtol=0, atol=1e-12)
+                np.testing.assert_allclose(f.variables[x + "_velocity"][k], vel[c].T, rtol=0, atol=1e-12)
+    with netcdf_file(base + "_field_stats.nc", "r", mmap=False) as f:
+        tt = np.array(f.variables["t"].data)
+        ke = np.array(f.variables["ke"].data)
+        assert len(ke) == len(rows) + 1              # the steps' rows, then the final time
+        np.testing.assert_array_equal(tt[:-1], np.array(rows)[:, 0])
+        np.testing.assert_allclose(ke[:-1], np.array(rows)[:, 2], rtol=1e-13)
+        assert tt[-1] == t
+    d.close()
