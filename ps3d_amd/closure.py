@@ -210,7 +210,12 @@ def _fold(nf, nc):
 def energy_spectrum(solver, ke):
     """Spectrum of the solver's current physical velocity (after vor2vel) in
     shells of K = sqrt(kx^2 + ky^2 + kz^2), normalised so sum(spec) dk = ke."""
-    vel = solver.get(G.VEL)
+    return velocity_spectrum(solver, solver.get(G.VEL), ke)
+
+
+def velocity_spectrum(solver, vel, ke):
+    """genspec's spectrum of a physical velocity (3 volumes) on solver's grid:
+    (shell wavenumbers k dk, spectrum), k = 0 .. kmax."""
     nx, ny, nz = solver.nx, solver.ny, solver.nz
     s = [solver.op(G.OP_FFTXYP2S, vel[c]) for c in range(3)]
     s[0] = solver.op(G.OP_FFTCOSINE, s[0])

@@ -16,6 +16,7 @@ import numpy as np
 from .gpu import DELTA, NDIAG, PRES, VEL, VOR, lib
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _ready = False
 
 
@@ -34,6 +35,7 @@ def _L():
         L.ps3d_nc_read_fields.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.c_int,
                                           C.c_int, C.c_void_p, _dp]
         L.ps3d_nc_records.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.ps3d_nc_grid.argtypes = [C.c_char_p, _ip, _dp, _dp]
         L.ps3d_nc_create_stats.argtypes = [C.c_char_p]
         L.ps3d_nc_append_stats.argtypes = [C.c_char_p, _dp, C.c_int]
         _ready = True
@@ -55,6 +57,15 @@ def records(path):
     t = C.c_double()
     _check(_L().ps3d_nc_records(_b(path), C.byref(n), C.byref(t)))
     return n.value, t.value
+
+
+def grid(path):
+    """(ncells, extent, origin) of a field file (get_netcdf_box)."""
+    n = np.zeros(3, dtype=np.int32)
+    e = np.zeros(3)
+    o = np.zeros(3)
+    _check(_L().ps3d_nc_grid(_b(path), n, e, o))
+    return [int(v) for v in n], list(e), list(o)
 
 
 def _resume(path):
