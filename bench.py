@@ -109,6 +109,7 @@ def ensemble_bench(G, steps, sep_steps):
             vors.append((gi, v))
             i += 1
     shapes = ens.tune(steps=2)
+    line_threads = ens.line_choices()
     ens.advance(2)                      # builds the graph
     t0 = time.perf_counter()
     ev_ms = ens.advance(steps, graph=True)
@@ -135,6 +136,7 @@ def ensemble_bench(G, steps, sep_steps):
         "steps": steps, "ms_per_step": 1e3 * el / steps, "event_ms_per_step": ev_ms / steps,
         "grid_point_updates_per_s": npts * steps / el,
         "z_shapes": {str(n): c for n, c in zip(ENSEMBLE_SIZES, shapes)},
+        "line_threads": {str(n): list(c) for n, c in zip(ENSEMBLE_SIZES, line_threads)},
         "step_roofline": {"bound": "hbm", "algo_bytes_per_step": b_step, "achieved": b_step * steps / el / 1e9,
                           "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": b_step * steps / el / PEAK_HBM},
         "separate": {"steps": sep_steps, "ms_per_step": 1e3 * el_sep / sep_steps,

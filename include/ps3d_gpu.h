@@ -243,11 +243,16 @@ int ps3d_gpu_ensemble_group(ps3d_ensemble *e, int g, ps3d_handle **h);
  * captured graph per step.  ms: kernel time (HIP events), may be NULL. */
 int ps3d_gpu_ensemble_advance(ps3d_ensemble *e, int nsteps, int use_graph, double *ms);
 /* Times `steps` steps of each group under each available z launch shape and
- * keeps the fastest (state restored afterwards).  choice[g] (may be NULL)
- * receives the shape code: 100..102 = one wavefront per column (register budget
- * variant), q < 100 = q quads per block of the quad-per-thread kernels. */
+ * keeps the fastest, then, one mode at a time, the block size of each
+ * register-holding line pass (state restored afterwards).  choice[g] (may be
+ * NULL) receives the shape code: 100..102 = one wavefront per column (register
+ * budget variant), q < 100 = q quads per block of the quad-per-thread kernels. */
 int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice);
 int ps3d_gpu_ensemble_set_choice(ps3d_ensemble *e, int g, int code);
+/* Block sizes (threads) the tuner chose for group g's register-holding line passes:
+ * threads[0] x products, [1] adapt-stage y, [2] x adapt (1024, 512 or 256: full,
+ * half or quarter tile depth).  No counterpart in the reference. */
+int ps3d_gpu_ensemble_line_choice(ps3d_ensemble *e, int g, int *threads);
 int ps3d_gpu_ensemble_destroy(ps3d_ensemble *e);
 
 #ifdef __cplusplus

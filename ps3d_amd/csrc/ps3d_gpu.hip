@@ -231,6 +231,8 @@ static void free_handle(ps3d_handle *h) {
 }
 
 // Builds tables, plans and launch shapes. alloc_fields = false for the 1-D helpers.
+static void apply_line_choice(ps3d_handle *h, int tm, int ty, int tx);
+
 static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, bool strict, int nbatch = 1) {
     h->p = *p;
     h->nbatch = nbatch;
@@ -341,16 +343,12 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
         // y pass, x adapt pass): 1024 threads on the standard tile, 512 on half of it,
         // 256 on a quarter.  Measured fastest: 512, 512, 256 at 256^3, 512 for all
         // three at 512^3, and 1024, 1024, 512 at 1024^3 (where a half tile holds two z
-        // levels).  PS3D_LINE_T="m,y,x" overrides.
+        // levels); the ensemble tuner picks 256 for all three at n <= 96.
+        // PS3D_LINE_T="m,y,x" overrides.
         int tm = nx <= 512 ? 512 : 1024, ty = ny <= 512 ? 512 : 1024, tx = nx <= 256 ? 256 : 512;
+        if (std::max(nx, ny) <= 96) tm = ty = tx = 256;
         if (const char *ev = std::getenv("PS3D_LINE_T")) std::sscanf(ev, "%d,%d,%d", &tm, &ty, &tx);
-        auto pick = [](int n, int T) { return (T != 1024 && ps3d::line_reg_ok(n, T)) ? T : 1024; };
-        h->rtm = pick(nx, tm);
-        h->rty = pick(ny, ty);
-        h->rtx = pick(nx, tx);
-        h->lxm = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, h->rtm);
-        h->lyr = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs, h->rty);
-        h->lxr = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, h->rtx);
+        apply_line_choice(h, tm, ty, tx);
     }
 
     HIPCHK(ps3d::configure_zpass());
@@ -358,8 +356,11 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     HIPCHK(ps3d::configure_linepass());
 
     const size_t nb = (size_t)nbatch;
-    h->b_party = (long)h->lyr.nother * h->lyr.nzt * RY_N;
-    h->b_partx = (long)h->lxr.nother * h->lxr.nzt * RX_N;
+    // per-tile partial sums sized for the shallowest tiles (2 levels), so that any
+    // block size of the adapt passes fits
+    const long nzt_max = (nz + 2) / 2;
+    h->b_party = (long)h->lyr.nother * nzt_max * RY_N;
+    h->b_partx = (long)h->lxr.nother * nzt_max * RX_N;
     h->b_roll = std::max(1, p->roll_mean_win_size);
     HIPCHK(hipMalloc((void **)&h->scal, sizeof(double) * SC_NSCAL * nb));
     HIPCHK(hipMemset(h->scal, 0, sizeof(double) * SC_NSCAL * nb));
@@ -394,6 +395,20 @@ static Batch batch(const ps3d_handle *h) {
     b.diag = h->b_diag;
     b.roll = h->b_roll;
     return b;
+}
+
+// Block sizes of the register-holding line modes (1024, 512 or 256 threads; a size
+// without a build for the line length falls back to 1024) and their tiles.
+static void apply_line_choice(ps3d_handle *h, int tm, int ty, int tx) {
+    const Grid &g = h->g;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    auto pick = [](int n, int T) { return (T != 1024 && ps3d::line_reg_ok(n, T)) ? T : 1024; };
+    h->rtm = pick(nx, tm);
+    h->rty = pick(ny, ty);
+    h->rtx = pick(nx, tx);
+    h->lxm = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, h->rtm);
+    h->lyr = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs, h->rty);
+    h->lxr = line_cfg(nx, nz, g.zs, ny, (long)ny * g.zs, (long)g.zs, h->rtx);
 }
 
 static LineArgs line_args(const ps3d_handle *h, int axis, int mode) {
@@ -1592,6 +1607,33 @@ int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice) {
             }
         }
         apply_z_choice(h, best);
+        // line passes: block size (tile depth) of each register-holding mode in turn
+        if (!st) {
+            int cur[3] = {h->rtm, h->rty, h->rtx};
+            for (int mode = 0; mode < 3 && !st; ++mode) {
+                int bestT = cur[mode];
+                float bt = 1e30f;
+                for (int T : {1024, 512, 256}) {
+                    int c[3] = {cur[0], cur[1], cur[2]};
+                    c[mode] = T;
+                    apply_line_choice(h, c[0], c[1], c[2]);
+                    if ((mode == 0 ? h->rtm : mode == 1 ? h->rty : h->rtx) != T) continue;   // no build
+                    if ((st = enqueue_steps(h, 1))) break;
+                    HIPCHK(hipEventRecord(h->ev0, h->stream));
+                    if ((st = enqueue_steps(h, steps))) break;
+                    HIPCHK(hipEventRecord(h->ev1, h->stream));
+                    HIPCHK(hipStreamSynchronize(h->stream));
+                    float ms = 0.0f;
+                    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+                    if (ms < bt) {
+                        bt = ms;
+                        bestT = T;
+                    }
+                }
+                cur[mode] = bestT;
+            }
+            apply_line_choice(h, cur[0], cur[1], cur[2]);
+        }
         for (size_t b = 0; b < nb; ++b)
             HIPCHK(hipMemcpyAsync(h->svor[0] + b * h->b_vol, bk + b * sv, sizeof(double) * sv, hipMemcpyDeviceToDevice,
                                   h->stream));
@@ -1603,6 +1645,15 @@ int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice) {
         if (choice) choice[gi] = best;
     }
     drop_graph(e);
+    return 0;
+}
+
+int ps3d_gpu_ensemble_line_choice(ps3d_ensemble *e, int g, int *threads) {
+    if (!e || !threads || g < 0 || g >= (int)e->g.size()) return fail(PS3D_E_ARG, "invalid argument");
+    const ps3d_handle *h = e->g[g];
+    threads[0] = h->rtm;
+    threads[1] = h->rty;
+    threads[2] = h->rtx;
     return 0;
 }
 

@@ -75,6 +75,7 @@ def lib():
         L.ps3d_gpu_ensemble_advance.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.ps3d_gpu_ensemble_tune.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.ps3d_gpu_ensemble_set_choice.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ps3d_gpu_ensemble_line_choice.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.ps3d_gpu_ensemble_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
@@ -328,9 +329,20 @@ class Ps3dEnsemble:
         return ms.value
 
     def tune(self, steps=2):
+        """z launch shape, then line-pass block sizes, per group; returns the z shape
+        codes (line_choices() gives the block sizes)."""
         ch = (C.c_int * len(self.groups))()
         _check(lib().ps3d_gpu_ensemble_tune(self.e, int(steps), ch))
         return list(ch)
+
+    def line_choices(self):
+        """(x products, adapt-stage y, x adapt) threads per block of each group."""
+        out = []
+        for g in range(len(self.groups)):
+            t = (C.c_int * 3)()
+            _check(lib().ps3d_gpu_ensemble_line_choice(self.e, g, t))
+            out.append(tuple(t))
+        return out
 
     def set_choice(self, g, code):
         _check(lib().ps3d_gpu_ensemble_set_choice(self.e, int(g), int(code)))
