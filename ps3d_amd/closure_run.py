@@ -33,9 +33,9 @@ def progress(msg):
     sys.stderr.flush()
 
 
-def params(n, limit=1.0e30):
+def params(n, limit=1.0e30, prediss=30.0):
     return G.make_params(n, n, n, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI), filtering="2/3-rule",
-                         stepper="impl-diff-rk4", nnu=3, prediss=30.0, pretype="vorch", length_scale="Kolmogorov",
+                         stepper="impl-diff-rk4", nnu=3, prediss=prediss, pretype="vorch", length_scale="Kolmogorov",
                          alpha=0.1, dt_fixed=0.0, limit=limit)
 
 
@@ -97,9 +97,9 @@ def spectrum_of(h, t):
     return CL.energy_spectrum(h, row[G.DIAG_NAMES.index("ke")])
 
 
-def evaluate(nf, nc, net, seeds, t_end, gains=(1.0,)):
-    """Coarse runs without the closure and with it at each output gain (the net's
-    output scaled by g), against the truncated fine run at t_end."""
+def evaluate(nf, nc, net, seeds, t_end, configs):
+    """Coarse runs, one per config (tag, coarse prediss, closure output gain or None
+    for no closure), against the truncated fine run at t_end."""
     out = []
     kcut = int(nc / 3)   # shells below the coarse 2/3 cutoff
     base_scale = np.array(net.out_scale, dtype=np.float32)
@@ -107,9 +107,8 @@ def evaluate(nf, nc, net, seeds, t_end, gains=(1.0,)):
         k0 = 5.0
         f = fine_run(nf, k0, s, limit=t_end)
         runs = {}
-        tags = ["no_closure"] + ["closure" if len(gains) == 1 else "gain_%g" % g for g in gains]
-        for tag in tags:
-            c = G.Ps3dGpu(params(nc, limit=t_end))
+        for tag, pd, _ in configs:
+            c = G.Ps3dGpu(params(nc, limit=t_end, prediss=pd))
             G.resample(c, f)                    # truncated initial state
             runs[tag] = c
         nfs = 0
@@ -122,9 +121,9 @@ def evaluate(nf, nc, net, seeds, t_end, gains=(1.0,)):
         G.resample(cref, f)
         _, ef = spectrum_of(cref, t_end)
         res = {"seed": s, "fine_steps": nfs, "fine_s": fine_s}
-        for i, (tag, c) in enumerate(runs.items()):
-            if i > 0:
-                net.out_scale = base_scale * np.float32(gains[i - 1])
+        for (tag, c), (_, _, gain) in zip(runs.items(), configs):
+            if gain is not None:
+                net.out_scale = base_scale * np.float32(gain)
                 net.attach(c)
             n = 0
             t0 = time.perf_counter()
@@ -145,7 +144,7 @@ def evaluate(nf, nc, net, seeds, t_end, gains=(1.0,)):
                 err = float("inf")
             res[tag] = {"steps": n, "s": el, "spectrum_error": err, "completed": ok}
             progress("seed %d %s: %d steps, error %.4f" % (s, tag, n, err))
-            if i > 0:
+            if gain is not None:
                 CL.ClosureNet.detach(c)
                 net.out_scale = base_scale
             c.close()
@@ -194,6 +193,9 @@ def main(argv=None):
     ap.add_argument("--gains", default="1.0",
                     help="output gains tried on the validation seeds (comma separated); the best is evaluated")
     ap.add_argument("--val-seeds", type=int, default=1)
+    ap.add_argument("--coarse-prediss", default="30",
+                    help="coarse-grid hyperdiffusion prediss values tried on the validation seeds "
+                         "(the closure may take over part of the coarse grid's dissipation)")
     ap.add_argument("--target", choices=["transfer", "transfer+diffusion"], default="transfer",
                     help="closure target: the subgrid nonlinear transfer, or also the difference of the "
                          "two grids' hyperdiffusion")
@@ -206,15 +208,20 @@ def main(argv=None):
     npts = a.coarse * a.coarse * (a.coarse + 1)
     gflop = CL.GFLOP_FWD_PER_POINT * npts
     gains = [float(g) for g in a.gains.split(",")]
+    pds = [float(v) for v in a.coarse_prediss.split(",")]
     val = None
-    gain = gains[0]
-    if len(gains) > 1:
-        # output gain chosen on validation seeds (never the evaluation seeds)
-        val = evaluate(a.fine, a.coarse, net, [8001 + i for i in range(a.val_seeds)], a.t_end, gains)
-        err = [np.mean([v["gain_%g" % g]["spectrum_error"] for v in val]) for g in gains]
-        gain = gains[int(np.argmin(err))]
-    net.out_scale = net.out_scale * np.float32(gain)
-    ev = evaluate(a.fine, a.coarse, net, [9001 + i for i in range(a.eval_seeds)], a.t_end)
+    gain, pd_c, pd_p = gains[0], pds[0], 30.0
+    if len(gains) > 1 or len(pds) > 1:
+        # closure gain and coarse prediss (and the best prediss without closure) chosen
+        # on validation seeds, never on the evaluation seeds
+        cfg = [("plain_p%g" % p, p, None) for p in pds] + \
+              [("closure_p%g_g%g" % (p, g), p, g) for p in pds for g in gains]
+        val = evaluate(a.fine, a.coarse, net, [8001 + i for i in range(a.val_seeds)], a.t_end, cfg)
+        err = {tag: np.mean([v[tag]["spectrum_error"] for v in val]) for tag, _, _ in cfg}
+        _, pd_c, gain = min((c for c in cfg if c[2] is not None), key=lambda c: err[c[0]])
+        _, pd_p, _ = min((c for c in cfg if c[2] is None), key=lambda c: err[c[0]])
+    cfg = [("no_closure", 30.0, None), ("no_closure_tuned", pd_p, None), ("closure", pd_c, gain)]
+    ev = evaluate(a.fine, a.coarse, net, [9001 + i for i in range(a.eval_seeds)], a.t_end, cfg)
     st = step_timing(a.fine, a.coarse, net)
     out = {
         "fine": a.fine, "coarse": a.coarse, "target": a.target, "snapshots": len(X), "data_s": t_data,
@@ -223,7 +230,7 @@ def main(argv=None):
         "forward_mfma_frac": gflop / fwd_ms * 1e12 / 2.5e15,
         "train_step_ms": train_ms,
         "step_ms": st, "speedup_per_step": st["fine"] / st["coarse_closure"],
-        "gain": gain, "validation": val,
+        "gain": gain, "coarse_prediss_closure": pd_c, "coarse_prediss_plain_tuned": pd_p, "validation": val,
         "eval": ev,
     }
     tf = [e["fine_s"] for e in ev]
@@ -231,6 +238,7 @@ def main(argv=None):
     out["speedup_per_time"] = float(np.mean(tf) / np.mean(tc))
     out["spectrum_error_closure"] = float(np.mean([e["closure"]["spectrum_error"] for e in ev]))
     out["spectrum_error_no_closure"] = float(np.mean([e["no_closure"]["spectrum_error"] for e in ev]))
+    out["spectrum_error_no_closure_tuned"] = float(np.mean([e["no_closure_tuned"]["spectrum_error"] for e in ev]))
     net.close()
     print(json.dumps(out))
     return out
