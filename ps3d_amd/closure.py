@@ -169,12 +169,27 @@ def hyperdiffusion_rate(solver):
     return pref * visc * (k[:, None] ** 2 + l[None, :] ** 2) ** solver.p.nnu
 
 
-def sample_from_fine(fine, coarse, tcoarse, diffusion=False):
+def dealias_mask(solver):
+    """The 2/3-rule mask the coarse step multiplies every tendency by (filx fily
+    filz, the lids taking filx fily), on the host layout (nx, ny, nz+1)."""
+    nx, ny, nz = solver.nx, solver.ny, solver.nz
+    kx = np.minimum(np.arange(nx), nx - np.arange(nx)) * (2 * np.pi / solver.p.extent[0])
+    ky = np.minimum(np.arange(ny), ny - np.arange(ny)) * (2 * np.pi / solver.p.extent[1])
+    kz = np.arange(nz + 1) * (np.pi / solver.p.extent[2])
+    fx = (kx <= 2.0 / 3.0 * kx.max()).astype(float)
+    fy = (ky <= 2.0 / 3.0 * ky.max()).astype(float)
+    fz = (kz <= 2.0 / 3.0 * kz.max()).astype(float)
+    fz[0] = fz[-1] = 1.0
+    return fx[:, None, None] * fy[None, :, None] * fz[None, None, :]
+
+
+def sample_from_fine(fine, coarse, tcoarse, diffusion=False, dealias=False):
     """One training pair at the current state of `fine` (its vor2vel and source
     run here): input = coarse (u, v, w, omega) of R(omega_f); target = R(S_f) -
     S_c(R(omega_f)) in physical space on the coarse points.  diffusion=True adds
     the difference of the two grids' hyperdiffusion at the coarse wavenumbers,
-    (rate_c - rate_f) R(omega_f), so the target is the whole missing tendency."""
+    (rate_c - rate_f) R(omega_f), so the target is the whole missing tendency.
+    dealias=True keeps only the modes the coarse step passes (its 2/3-rule mask)."""
     fine.vor2vel()
     if diffusion:
         fine.adapt(fine.t)
@@ -193,6 +208,8 @@ def sample_from_fine(fine, coarse, tcoarse, diffusion=False):
         rate_f = hyperdiffusion_rate(fine)[np.ix_(_fold(fine.nx, coarse.nx), _fold(fine.ny, coarse.ny))]
         rate = hyperdiffusion_rate(coarse) - rate_f
         d = d + rate[None, :, :, None] * coarse.get(G.SVOR)
+    if dealias:
+        d = d * dealias_mask(coarse)[None]
     tau = np.stack([coarse.op(G.OP_COMBINE_PHYS, d[c]) for c in range(3)])
     x = to_points(np.concatenate([vel, vor]))
     return x.astype(np.float32), to_points(tau).astype(np.float32)

@@ -6,11 +6,15 @@ compare the coarse run with and without it against the filtered fine run.
 Fine runs: random-phase isotropic initial vorticity, E(k) ~ k^4 exp(-2 (k/k0)^2)
 with k0 ~ U[3, 8] per run (seeded), made on 128^3 and zero-padded to the fine
 grid; the plain FP64 step (impl-diff-rk4, 2/3-rule, nnu 3, prediss 30, vorch).
-Snapshots every `every` steps after `spinup` steps.  Held-out seeds are run to the
-same physical time on the fine grid (truncated to the coarse one for its spectrum)
-and on the coarse grid from the truncated initial state, with and without the
-closure; the genspec energy spectra are compared below the coarse 2/3 cutoff.
-Prints one JSON object.
+Snapshots every `every` steps after `spinup` steps; the training target keeps the
+modes the coarse step's 2/3-rule mask passes.  On validation seeds the closure's
+output gain and the coarse grid's prediss are chosen (and, for the comparison, the
+best prediss without closure); held-out seeds are then run to the same physical
+time on the fine grid (truncated to the coarse one for its spectrum) and on the
+coarse grid from the truncated initial state: without closure at prediss 30,
+without closure at its best prediss, and with the closure.  The genspec energy
+spectra are compared below the coarse 2/3 cutoff.  Progress goes to stderr; the
+result is one JSON object on stdout.
 """
 import argparse
 import json
@@ -48,7 +52,7 @@ def fine_run(nf, k0, seed, limit=1.0e30):
     return h
 
 
-def dataset(nf, nc, runs, snaps, every, spinup, seed0, diffusion=False):
+def dataset(nf, nc, runs, snaps, every, spinup, seed0, diffusion=False, dealias=False):
     rng = np.random.default_rng(seed0)
     coarse, tco = G.Ps3dGpu(params(nc)), G.Ps3dGpu(params(nc))
     X, T = [], []
@@ -56,7 +60,7 @@ def dataset(nf, nc, runs, snaps, every, spinup, seed0, diffusion=False):
         f = fine_run(nf, float(rng.uniform(3.0, 8.0)), seed0 + r)
         f.advance_n(spinup)
         for s in range(snaps):
-            x, t = CL.sample_from_fine(f, coarse, tco, diffusion)
+            x, t = CL.sample_from_fine(f, coarse, tco, diffusion, dealias)
             X.append(x)
             T.append(t)
             f.advance_n(every)
@@ -190,18 +194,22 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval-seeds", type=int, default=2)
     ap.add_argument("--t-end", type=float, default=0.5)
-    ap.add_argument("--gains", default="1.0",
+    ap.add_argument("--gains", default="0.25,0.5,1.0",
                     help="output gains tried on the validation seeds (comma separated); the best is evaluated")
-    ap.add_argument("--val-seeds", type=int, default=1)
-    ap.add_argument("--coarse-prediss", default="30",
+    ap.add_argument("--val-seeds", type=int, default=2)
+    ap.add_argument("--coarse-prediss", default="30,7.5,3.75",
                     help="coarse-grid hyperdiffusion prediss values tried on the validation seeds "
                          "(the closure may take over part of the coarse grid's dissipation)")
+    ap.add_argument("--no-dealias-target", dest="dealias_target", action="store_false",
+                    help="train on the whole truncated target, including the modes the coarse step's "
+                         "2/3-rule mask removes (by default those are dropped)")
     ap.add_argument("--target", choices=["transfer", "transfer+diffusion"], default="transfer",
                     help="closure target: the subgrid nonlinear transfer, or also the difference of the "
                          "two grids' hyperdiffusion")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
-    X, T = dataset(a.fine, a.coarse, a.runs, a.snaps, a.every, a.spinup, 500, a.target == "transfer+diffusion")
+    X, T = dataset(a.fine, a.coarse, a.runs, a.snaps, a.every, a.spinup, 500, a.target == "transfer+diffusion",
+                   a.dealias_target)
     t_data = time.perf_counter() - t0
     net, hist, t_train = train(a.coarse, X, T, a.epochs, a.lr, 0)
     fwd_ms, train_ms = net.timing(20)
@@ -224,7 +232,8 @@ def main(argv=None):
     ev = evaluate(a.fine, a.coarse, net, [9001 + i for i in range(a.eval_seeds)], a.t_end, cfg)
     st = step_timing(a.fine, a.coarse, net)
     out = {
-        "fine": a.fine, "coarse": a.coarse, "target": a.target, "snapshots": len(X), "data_s": t_data,
+        "fine": a.fine, "coarse": a.coarse, "target": a.target, "dealias_target": a.dealias_target,
+        "snapshots": len(X), "data_s": t_data,
         "epochs": a.epochs, "train_s": t_train, "loss_first": hist[0], "loss_last": hist[-1],
         "forward_ms": fwd_ms, "forward_gflop": gflop, "forward_tflops": gflop / fwd_ms,
         "forward_mfma_frac": gflop / fwd_ms * 1e12 / 2.5e15,
