@@ -5,8 +5,10 @@ runs, training, and the genspec-style energy spectrum used to evaluate it.
 Training target: tau = R(S_f) - S_c(R(omega_f)), where S is ps3d's vorticity
 tendency (src/inversion/inversion.f90:298-388), R the sharp spectral truncation
 of a fine grid onto the coarse one (ps3d_gpu_resample) and omega_f a fine-grid
-state; input: the coarse u, v, w, xi, eta, zeta.  Both are physical fields on the
-coarse points, scaled to unit rms per group (velocity, vorticity, tendency).
+state, optionally restricted to the modes the coarse step's 2/3-rule mask passes
+(the experiment driver does so); input: the coarse u, v, w, xi, eta, zeta.  Both
+are physical fields on the coarse points, scaled to unit rms per group (velocity,
+vorticity, tendency).
 """
 import ctypes as C
 import math
