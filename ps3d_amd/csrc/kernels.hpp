@@ -39,7 +39,7 @@ struct Batch {
     int nb;                          // instances (gridDim.y); 0 or 1: a single grid
     long vol;                        // field pool
     long part;                       // per-block partials of a line pass
-    long party, partx, diag, roll;   // finishing reductions
+    long party, partx, partm, diag, roll;   // finishing reductions
 };
 
 // 1-D tables on the device
@@ -168,9 +168,11 @@ enum LineMode {
     LM_DIFF = 5       // spectral derivative along the line (diffx / diffy), no transform
 };
 
-// Reduction slots written per block by LM_INV_Y1 (sums) and LM_XADAPT (maxima / sums)
+// Reduction slots written per block by LM_INV_Y1 (sums), LM_XADAPT (maxima / sums)
+// and, at the adapt stage, LM_XMAIN (velocity maxima)
 enum { RY_UU = 0, RY_WW, RY_UW, RY_MX, RY_MY, RY_MZ, RY_N };
 enum { RX_W2MAX = 0, RX_GG, RX_USGG, RX_LSGG, RX_UMAX, RX_VMAX, RX_WMAX, RX_L1, RX_L2, RX_N };
+enum { RM_UMAX = 0, RM_VMAX, RM_WMAX, RM_N };
 
 struct LineArgs {
     Grid g;
@@ -188,7 +190,8 @@ struct LineArgs {
     int phys_in;       // LM_XADAPT / LM_XMAIN: inputs already physical in x
     const double *in[12];
     double *out[12];
-    double *part;      // per-block partial results
+    double *part;      // per-block partial results (LM_XMAIN: velocity maxima when set)
+    int velmax_given;  // LM_XADAPT: the velocity maxima come from the LM_XMAIN pass
     const double *scal;
     double fcor[3];
     double ncelli;
@@ -211,6 +214,7 @@ hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream);
 struct FinArgs {
     const double *party;  int nby;
     const double *partx;  int nbx;
+    const double *partm;  int nbm;   // velocity maxima of LM_XMAIN, or nullptr
     double *scal;
     double *diag;         // series base (row SC_STEP), or nullptr
     int max_rows;

@@ -731,6 +731,45 @@ k_line_reg(LineArgs a) {
         }
         nn_export(t, a, o, 4);
         nn_export(t, a, o2, 5);
+        if (a.part) {
+            // adapt stage: signed maxima of u, v, w (advance.f90:319-321), one partial
+            // per block, so that the x adapt pass need not load the velocity again
+            // (the tile buffers are free here: the last transform ended on a barrier)
+            double um = -INFINITY, vm = -INFINITY, wm = -INFINITY;
+#pragma unroll
+            for (int k = 0; k < kEpt; ++k)
+                for (int h = 0; h < 2; ++h)
+                    if (reg_valid(t, k, h, nz)) {
+                        um = fmax(um, h ? u[k].y : u[k].x);
+                        vm = fmax(vm, h ? v[k].y : v[k].x);
+                        wm = fmax(wm, h ? w[k].y : w[k].x);
+                    }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) {
+                um = fmax(um, __shfl_xor(um, m));
+                vm = fmax(vm, __shfl_xor(vm, m));
+                wm = fmax(wm, __shfl_xor(wm, m));
+            }
+            const int wv = threadIdx.x / 64;
+            if ((threadIdx.x & 63) == 0) {
+                scratch[RM_N * wv + RM_UMAX] = um;
+                scratch[RM_N * wv + RM_VMAX] = vm;
+                scratch[RM_N * wv + RM_WMAX] = wm;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (int i = 1; i < T / 64; ++i) {
+                    um = fmax(um, scratch[RM_N * i + RM_UMAX]);
+                    vm = fmax(vm, scratch[RM_N * i + RM_VMAX]);
+                    wm = fmax(wm, scratch[RM_N * i + RM_WMAX]);
+                }
+                double *pm = a.part + (size_t)blockIdx.y * a.bt.part + (size_t)tile_index() * RM_N;
+                pm[RM_UMAX] = um;
+                pm[RM_VMAX] = vm;
+                pm[RM_WMAX] = wm;
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int k = 0; k < kEpt; ++k) {
             const double er = o[k].x + a.fcor[1], ei = o[k].y + a.fcor[1];
@@ -802,26 +841,35 @@ k_line_reg(LineArgs a) {
         if constexpr (N > 0) {
             if (!a.phys_in) {
                 pf_load(p0, t, a, a.in[0]);
-                pf_load(p1, t, a, a.in[1]);
+                pf_load(p1, t, a, a.in[a.velmax_given ? 2 : 1]);
             }
         }
-        get2(0, 1, false, ux, vy, 2, 3);       // u, v (maxima)
+        if (a.velmax_given) {
+            // the products pass has taken the velocity maxima: (u_x, w_x) (u_y, v_y)
+            // (w_y, eta) (xi, zeta), eleven tile loads in all instead of fourteen
+            get2(0, 2, true, ux, wx, 6, 7);    // du/dx, dw/dx
+            get2(6, 7, false, uy, vy, 8, 4);   // du/dy, dv/dy
+            get2(8, 4, false, wy, et, 3, 5);   // dw/dy, eta
+            get2(3, 5, false, xi, ze, 9, 10);  // xi, zeta
+        } else {
+            get2(0, 1, false, ux, vy, 2, 3);   // u, v (maxima)
 #pragma unroll
-        for (int k = 0; k < kEpt; ++k)
-            for (int h = 0; h < 2; ++h)
-                if (reg_valid(t, k, h, nz)) {
-                    umax = fmax(umax, h ? ux[k].y : ux[k].x);
-                    vmax = fmax(vmax, h ? vy[k].y : vy[k].x);
-                }
-        get2(2, 3, false, wx, xi, 0, 2);       // w (maximum), xi
+            for (int k = 0; k < kEpt; ++k)
+                for (int h = 0; h < 2; ++h)
+                    if (reg_valid(t, k, h, nz)) {
+                        umax = fmax(umax, h ? ux[k].y : ux[k].x);
+                        vmax = fmax(vmax, h ? vy[k].y : vy[k].x);
+                    }
+            get2(2, 3, false, wx, xi, 0, 2);   // w (maximum), xi
 #pragma unroll
-        for (int k = 0; k < kEpt; ++k)
-            for (int h = 0; h < 2; ++h)
-                if (reg_valid(t, k, h, nz)) wmax = fmax(wmax, h ? wx[k].y : wx[k].x);
-        get2(0, 2, true, ux, wx, 6, 7);        // du/dx, dw/dx
-        get2(6, 7, false, uy, vy, 8, 4);       // du/dy, dv/dy
-        get2(8, 4, false, wy, et, 5, -1);      // dw/dy, eta
-        get2(5, -1, false, ze, ze, 9, 10);     // zeta
+            for (int k = 0; k < kEpt; ++k)
+                for (int h = 0; h < 2; ++h)
+                    if (reg_valid(t, k, h, nz)) wmax = fmax(wmax, h ? wx[k].y : wx[k].x);
+            get2(0, 2, true, ux, wx, 6, 7);    // du/dx, dw/dx
+            get2(6, 7, false, uy, vy, 8, 4);   // du/dy, dv/dy
+            get2(8, 4, false, wy, et, 5, -1);  // dw/dy, eta
+            get2(5, -1, false, ze, ze, 9, 10); // zeta
+        }
         // S11 = u_x, S12 = u_y + zeta/2, S13 = w_x + eta/2, S22 = v_y,
         // S23 = w_y - xi/2, S33 = -(u_x + v_y)  (advance.f90:225-257).  The block's
         // maximum of the Jacobi values is found by running Jacobi only where the
@@ -1038,6 +1086,15 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_x(FinArgs a) {
         wm = fmax(wm, p[RX_WMAX]);
         l1 += p[RX_L1];
         l2 += p[RX_L2];
+    }
+    if (a.partm) {
+        const double *partm = a.partm + (size_t)blockIdx.y * a.bt.partm;
+        for (int b = threadIdx.x; b < a.nbm; b += blockDim.x) {
+            const double *p = partm + (size_t)b * RM_N;
+            um = fmax(um, p[RM_UMAX]);
+            vm = fmax(vm, p[RM_VMAX]);
+            wm = fmax(wm, p[RM_WMAX]);
+        }
     }
     w2 = block_max(w2, red);
     gg = block_max(gg, red);

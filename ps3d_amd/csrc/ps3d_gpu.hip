@@ -58,7 +58,7 @@ struct ps3d_handle {
     size_t nvol = 0;               // doubles per padded volume
     double *pool = nullptr;
     double *svor[3]{}, *qdi[3]{}, *qdf[3]{}, *svel[3]{}, *vel[3]{}, *vor[3]{}, *grad[3]{}, *mid[3]{};
-    double *scal = nullptr, *party = nullptr, *partx = nullptr, *roll = nullptr;
+    double *scal = nullptr, *party = nullptr, *partx = nullptr, *partm = nullptr, *roll = nullptr;
     double *pres = nullptr, *delta = nullptr;   // output-step derived fields, allocated on first use
     double *diag = nullptr;
     int diag_cap = 0;
@@ -78,7 +78,7 @@ struct ps3d_handle {
     // batched instances of this grid (ensemble group): instance b's data sit at
     // b * stride from instance 0; launches of the stepping kernels cover all of them
     int nbatch = 1;
-    long b_vol = 0, b_party = 0, b_partx = 0, b_roll = 0, b_diag = 0;
+    long b_vol = 0, b_party = 0, b_partx = 0, b_partm = 0, b_roll = 0, b_diag = 0;
     std::vector<char> inst_set;    // instances given an initial vorticity
     ps3d_cnn *cnn = nullptr;       // attached closure (owned by the caller)
     float cnn_in[6]{}, cnn_out[3]{};
@@ -213,6 +213,7 @@ static void free_handle(ps3d_handle *h) {
     hipFree(h->scal);
     hipFree(h->party);
     hipFree(h->partx);
+    hipFree(h->partm);
     hipFree(h->roll);
     hipFree(h->diag);
     hipFree(h->rkx);
@@ -361,11 +362,13 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     const long nzt_max = (nz + 2) / 2;
     h->b_party = (long)h->lyr.nother * nzt_max * RY_N;
     h->b_partx = (long)h->lxr.nother * nzt_max * RX_N;
+    h->b_partm = (long)h->lxm.nother * nzt_max * RM_N;
     h->b_roll = std::max(1, p->roll_mean_win_size);
     HIPCHK(hipMalloc((void **)&h->scal, sizeof(double) * SC_NSCAL * nb));
     HIPCHK(hipMemset(h->scal, 0, sizeof(double) * SC_NSCAL * nb));
     HIPCHK(hipMalloc((void **)&h->party, sizeof(double) * h->b_party * nb));
     HIPCHK(hipMalloc((void **)&h->partx, sizeof(double) * h->b_partx * nb));
+    HIPCHK(hipMalloc((void **)&h->partm, sizeof(double) * h->b_partm * nb));
     HIPCHK(hipMalloc((void **)&h->roll, sizeof(double) * h->b_roll * nb));
     if (alloc_fields) {
         const size_t nv = 24;
@@ -392,6 +395,7 @@ static Batch batch(const ps3d_handle *h) {
     b.part = 0;
     b.party = h->b_party;
     b.partx = h->b_partx;
+    b.partm = h->b_partm;
     b.diag = h->b_diag;
     b.roll = h->b_roll;
     return b;
@@ -597,9 +601,11 @@ static int inverse_adapt_stage(ps3d_handle *h, bool project) {
     return 0;
 }
 
-static int x_adapt(ps3d_handle *h, int phys_in) {
+// velmax_given: the products pass of this stage has written the velocity maxima
+static int x_adapt(ps3d_handle *h, int phys_in, bool velmax_given = false) {
     LineArgs x = line_args(h, 0, LM_XADAPT);
     x.phys_in = phys_in;
+    x.velmax_given = velmax_given;
     for (int c = 0; c < 3; ++c) {
         x.in[c] = h->vel[c];
         x.in[3 + c] = h->vor[c];
@@ -612,16 +618,26 @@ static int x_adapt(ps3d_handle *h, int phys_in) {
     int st = run_line(h, x);
     prof_end(h);
     if (st) return st;
+    FinArgs f = fin_args(h);
+    if (velmax_given) {
+        f.partm = h->partm;
+        f.nbm = h->lxm.nother * h->lxm.nzt;
+    }
     prof_begin(h, PS3D_K_REDUCE);
-    HIPCHK(launch_fin_x(fin_args(h), h->stream));
+    HIPCHK(launch_fin_x(f, h->stream));
     prof_end(h);
     return 0;
 }
 
-// x pass (u x omega) + y forward: r, q, p left semi-spectral in dst[]
-static int products(ps3d_handle *h, int phys_in, double *const *dst, bool nn_export = false) {
+// x pass (u x omega) + y forward: r, q, p left semi-spectral in dst[]; with
+// velmax, the x pass also writes the per-wave velocity maxima into partm
+static int products(ps3d_handle *h, int phys_in, double *const *dst, bool nn_export = false, bool velmax = false) {
     LineArgs x = line_args(h, 0, LM_XMAIN);
     x.phys_in = phys_in;
+    if (velmax) {
+        x.part = h->partm;
+        x.bt.part = h->b_partm;
+    }
     for (int c = 0; c < 3; ++c) {
         x.in[c] = h->vel[c];
         x.in[3 + c] = h->vor[c];
@@ -644,9 +660,14 @@ static int products(ps3d_handle *h, int phys_in, double *const *dst, bool nn_exp
 // One fused RK stage (impl_rk4) or cn2 iteration.  stage 1 includes adapt.
 static int fused_stage(ps3d_handle *h, int mode, int stage) {
     int st;
+    double *const *rqp = h->vel;
     if (stage == 1) {
         if ((st = inverse_adapt_stage(h, true))) return st;
-        if ((st = x_adapt(h, 0))) return st;
+        // the products pass goes first and also takes the velocity maxima; its output
+        // goes to svel (free after the y pass), as the adapt pass still reads vel
+        if ((st = products(h, 0, h->svel, true, true))) return st;
+        if ((st = x_adapt(h, 0, true))) return st;
+        rqp = h->svel;
     } else {
         ZInvArgs z = zinv_args(h);
         prof_begin(h, PS3D_K_ZINV);
@@ -658,9 +679,9 @@ static int fused_stage(ps3d_handle *h, int mode, int stage) {
         st = line_fields(h, 1, LM_INV, 6, in, out);
         prof_end(h);
         if (st) return st;
+        if ((st = products(h, 0, h->vel))) return st;
     }
-    if ((st = products(h, 0, h->vel, stage == 1))) return st;
-    ZTendArgs t = ztend_args(h, mode, stage, h->vel);
+    ZTendArgs t = ztend_args(h, mode, stage, rqp);
     if (h->cnn) {
         // closure: net on (u, v, w, omega) at the step's first stage -> physical
         // tendency in grad, decomposed like a physical field into mid and added to the
@@ -755,6 +776,7 @@ static ps3d_handle make_view(const ps3d_handle *h, int b) {
     v.scal = h->scal + (size_t)b * SC_NSCAL;
     v.party = h->party + (size_t)b * h->b_party;
     v.partx = h->partx + (size_t)b * h->b_partx;
+    v.partm = h->partm + (size_t)b * h->b_partm;
     v.roll = h->roll + (size_t)b * h->b_roll;
     v.diag = h->diag ? h->diag + (size_t)b * h->b_diag : nullptr;
     v.diag_cap = h->diag_cap;
@@ -1070,7 +1092,9 @@ int ps3d_gpu_profile_kernels(ps3d_handle *h, int nsteps, double *ms, int *launch
         // z forward + update, per stage: rk4 15, 15, 15, 9 ; cn2 12, 9, 9
         bytes[PS3D_K_ZTEND] = (cn2 ? 10.0 : 13.5) * vol;
         bytes[PS3D_K_YINV_ADAPT] = 21.0 * vol;  // 9 R + 12 W
-        bytes[PS3D_K_XADAPT] = 12.0 * vol;      // 12 R
+        // x adapt in a step: 11 R (u, w for their x derivatives, the y derivatives,
+        // omega, the midpoints); the velocity maxima come from the products pass
+        bytes[PS3D_K_XADAPT] = 11.0 * vol;
         bytes[PS3D_K_REDUCE] = 0.0;
         bytes[PS3D_K_MEAN] = 0.0;
     }
