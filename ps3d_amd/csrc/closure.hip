@@ -564,6 +564,8 @@ int ps3d_cnn_create(int nx, int ny, int nz, ps3d_cnn **out) {
     if (e == hipSuccess) e = hipMemset(c->m, 0, o * sizeof(float));
     if (e == hipSuccess) e = hipMemset(c->v, 0, o * sizeof(float));
     if (e == hipSuccess) e = hipMemset(c->loss_acc, 0, sizeof(float));
+    // the clears ran on the null stream; the net's kernels run on non-blocking streams
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         free_cnn(c);
         return cfail(PS3D_E_NOMEM, std::string("closure allocation: ") + hipGetErrorString(e));

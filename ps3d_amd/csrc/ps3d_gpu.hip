@@ -365,7 +365,9 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     h->b_partm = (long)h->lxm.nother * nzt_max * RM_N;
     h->b_roll = std::max(1, p->roll_mean_win_size);
     HIPCHK(hipMalloc((void **)&h->scal, sizeof(double) * SC_NSCAL * nb));
-    HIPCHK(hipMemset(h->scal, 0, sizeof(double) * SC_NSCAL * nb));
+    // clears go on the handle's stream: the stream is non-blocking, so a clear on
+    // the null stream would not be ordered before the handle's first kernels
+    HIPCHK(hipMemsetAsync(h->scal, 0, sizeof(double) * SC_NSCAL * nb, h->stream));
     HIPCHK(hipMalloc((void **)&h->party, sizeof(double) * h->b_party * nb));
     HIPCHK(hipMalloc((void **)&h->partx, sizeof(double) * h->b_partx * nb));
     HIPCHK(hipMalloc((void **)&h->partm, sizeof(double) * h->b_partm * nb));
@@ -376,12 +378,13 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
         const size_t bytes = sizeof(double) * h->nvol * nv * nb;
         hipError_t e = hipMalloc((void **)&h->pool, bytes);
         if (e != hipSuccess) return fail(PS3D_E_NOMEM, "device allocation of the field pool failed");
-        HIPCHK(hipMemset(h->pool, 0, bytes));
+        HIPCHK(hipMemsetAsync(h->pool, 0, bytes, h->stream));
         double *q = h->pool;
         auto take3 = [&](double **d) { for (int c = 0; c < 3; ++c) { d[c] = q; q += h->nvol; } };
         take3(h->svor); take3(h->qdi); take3(h->qdf); take3(h->svel);
         take3(h->vel); take3(h->vor); take3(h->grad); take3(h->mid);
     }
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 

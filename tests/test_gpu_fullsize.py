@@ -1,0 +1,147 @@
+"""The full 256^3 workload of the benchmark (2 pi x 2 pi x pi box, 2/3-rule,
+impl-diff-rk4, nnu = 3, prediss = 30, vorch prefactor, adaptive dt) at its real
+size:
+
+* one step of the benchmark's initial condition against the oracle (a single
+  step of the host restatement takes seconds on 16 threads);
+* vor2vel of the benchmark's Beltrami modes against their closed-form velocity,
+  as the reference's test_vor2vel_1.f90 checks on a 32^3 grid;
+* a Beltrami flow (modes sharing alpha = |k| and k^2 + l^2, so u x omega = 0)
+  decays by exp(-pref dt visc (k^2 + l^2)^nnu) per step: vdiss = dt/2 pref vhdis,
+  vhdis = visc (k^2 + l^2)^nnu (impl_rk4.f90:37-52, inversion_utils.f90:188-218).
+  ps3d projects the vorticity with central z differences, so the field stays
+  proportional to its start only to O(dz^2); the factor of each step is exact.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from ps3d_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+N = 256
+PI = math.pi
+
+
+@pytest.fixture(scope="module")
+def G():
+    from ps3d_amd import gpu
+    gpu.lib()
+    return gpu
+
+
+def params(mod):
+    return mod.make_params(N, N, N, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI),
+                           filtering="2/3-rule", stepper="impl-diff-rk4", nnu=3, prediss=30.0,
+                           pretype="vorch", length_scale="Kolmogorov", alpha=0.1, dt_fixed=0.0,
+                           limit=1.0e30)
+
+
+def beltrami(modes, seed):
+    """Velocity and vorticity of ics.beltrami_multimode(noise=0) with these modes."""
+    rng = np.random.default_rng(seed)
+    phases = rng.uniform(0.0, 2 * PI, size=len(modes))
+    x, y, z = ics.grid(N, N, N, (0.0, 0.0, 0.0), (2 * PI, 2 * PI, PI))
+    vel = np.zeros((3, N, N, N + 1))
+    vor = np.zeros((3, N, N, N + 1))
+    for (k, l, m), ph in zip(modes, phases):
+        u, w = ics.beltrami_mode(x, y, z, float(k), float(l), float(m), ph)
+        vel += u
+        vor += w
+    return vel, vor
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def test_one_benchmark_step_matches_oracle(oracle_mod, G):
+    vor0 = ics.beltrami_multimode(N, N, N, seed=1)
+    d = G.Ps3dGpu(params(G))
+    d.set_vorticity(vor0)
+    print("GPU step at %d^3" % N, flush=True)
+    row = d.advance_n(1)[0]
+    d.vor2vel()
+    oracle_mod.set_threads(min(16, os.cpu_count() or 1))
+    o = oracle_mod.Oracle(params(oracle_mod))
+    o.set_vorticity(vor0)
+    print("oracle step at %d^3" % N, flush=True)
+    orow = o.advance_n(1)[0]
+    o.vor2vel()
+    for j, name in enumerate(G.DIAG_NAMES):
+        assert abs(row[j] - orow[j]) <= 1.0e-12 * max(1.0, abs(orow[j])), (name, row[j], orow[j])
+    for which in (G.VOR, G.VEL, G.SVOR):
+        err = rel(d.get(which), o.get(which))
+        print("field", which, err, flush=True)
+        assert err < 1.0e-12
+    o.close()
+    d.close()
+
+
+def test_vor2vel_of_the_benchmark_beltrami_modes(G):
+    vel_ref, vor = beltrami(ics.BELTRAMI_MODES, seed=1)
+    np.testing.assert_array_equal(vor, ics.beltrami_multimode(N, N, N, seed=1, noise=0.0))
+    d = G.Ps3dGpu(params(G))
+    print("vor2vel at %d^3" % N, flush=True)
+    d.set_vorticity(vor)
+    d.vor2vel()
+    err = rel(d.get(G.VEL), vel_ref)
+    print("relative velocity error", err, flush=True)
+    assert err < 1.0e-13
+    d.close()
+
+
+def test_beltrami_flow_decays_in_closed_form(G):
+    # alpha = 9 and k^2 + l^2 = 80 for both modes
+    modes = ((8, 4, 1), (4, 8, 1))
+    _, vor0 = beltrami(modes, seed=7)
+    d = G.Ps3dGpu(params(G))
+    d.set_vorticity(vor0)
+    visc, _ = d.viscosity()
+    d.vor2vel()
+    prev = d.get(G.VOR)
+    # the kinetic energy of the first adapt is the closed form's <|u|^2>/2
+    vel0 = vor0 / 9.0
+    w = np.ones(N + 1)
+    w[0] = w[-1] = 0.5
+    ke0 = 0.5 * float(np.sum(np.sum(vel0 ** 2, axis=0) * w)) / (N * N * N)
+    for s in range(3):
+        print("step", s, flush=True)
+        row = d.advance_n(1)[0]
+        if s == 0:
+            ke = row[G.DIAG_NAMES.index("ke")]
+            assert abs(ke - ke0) <= 1.0e-11 * ke0, (ke, ke0)
+        dt = row[G.DIAG_NAMES.index("dt")]
+        pref = row[G.DIAG_NAMES.index("prefactor")]
+        pred = math.exp(-visc * 80.0 ** 3 * pref * dt)
+        d.vor2vel()
+        vor = d.get(G.VOR)
+        c = float(np.vdot(vor, prev) / np.vdot(prev, prev))
+        resid = rel(vor, c * prev)
+        print("decay", c, "closed form", pred, "residual", resid, flush=True)
+        assert 1.0 - pred > 1.0e-9          # the step's diffusion is well above the tolerance
+        assert abs(c / pred - 1.0) < 1.0e-10
+        # the O(dz^2) of the z-difference projection: 4e-6, 1e-6 at 64^3, 128^3
+        assert resid < 1.0e-6
+        # still Beltrami to the same order: u = omega / 9 (3e-5, 7.5e-6 at 64^3, 128^3)
+        assert rel(d.get(G.VEL), vor / 9.0) < 1.0e-5
+        prev = vor
+    d.close()
+
+
+def test_a_later_handle_computes_what_the_first_did(G):
+    """The first operation of a handle created after another one was closed gives
+    the same bits: the set-up clears of the field pool are ordered before the
+    handle's own work, whatever memory the allocator hands back."""
+    f = ics.beltrami_multimode(N, N, N, seed=3)[0]
+    out = []
+    for h in range(3):
+        d = G.Ps3dGpu(params(G))
+        print("handle", h, flush=True)
+        out.append(d.fftxyp2s(f))
+        d.close()
+    for h in (1, 2):
+        np.testing.assert_array_equal(out[h], out[0])
