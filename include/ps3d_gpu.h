@@ -117,7 +117,8 @@ const char *ps3d_gpu_version(void);
 
 /* = init_inversion + initialise_fft (inversion_utils.f90:222-372, sta3dfft.f90:53-110)
  * on the current HIP device. Rejects lengths with a prime factor other than 2 and 3
- * (nx, ny even; nz even) with PS3D_E_FACTOR. */
+ * (nx, ny even; nz even) with PS3D_E_FACTOR.  The handle owns a non-blocking stream;
+ * its device memory is cleared and the set-up complete when this returns. */
 int ps3d_gpu_create(const ps3d_params *p, ps3d_handle **h);
 /* = finalise_inversion (inversion_utils.f90:459-480). */
 int ps3d_gpu_destroy(ps3d_handle *h);
