@@ -1045,6 +1045,28 @@ hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
 // one block per instance; 1024 threads read the per-tile partials
 constexpr int kFinThreads = 1024;
 
+// NV values per thread reduced over the block (MAX: fmax, else sum) for thread 0:
+// shuffles inside each wave, then one pass over the per-wave results in LDS (two
+// barriers in all, where one block_sum/block_max per value takes ten each).
+template <int NV, bool MAX>
+__device__ inline void fin_reduce(double (&v)[NV], double *red) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const double o = __shfl_xor(v[q], m);
+            v[q] = MAX ? fmax(v[q], o) : v[q] + o;
+        }
+    if (lane == 0)
+        for (int q = 0; q < NV; ++q) red[wv * NV + q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 1; i < nw; ++i)
+            for (int q = 0; q < NV; ++q) v[q] = MAX ? fmax(v[q], red[i * NV + q]) : v[q] + red[i * NV + q];
+    __syncthreads();
+}
+
 __global__ void __launch_bounds__(kFinThreads) k_fin_y(FinArgs a) {
     __shared__ double red[kFinThreads];
     const double *party = a.party + (size_t)blockIdx.y * a.bt.party;
@@ -1053,8 +1075,8 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_y(FinArgs a) {
     for (int q = 0; q < RY_N; ++q) s[q] = 0.0;
     for (int b = threadIdx.x; b < a.nby; b += blockDim.x)
         for (int q = 0; q < RY_N; ++q) s[q] += party[(size_t)b * RY_N + q];
-    double tot[RY_N];
-    for (int q = 0; q < RY_N; ++q) tot[q] = block_sum(s[q], red);
+    fin_reduce<RY_N, false>(s, red);
+    const double(&tot)[RY_N] = s;
     if (threadIdx.x == 0) {
         // get_kinetic_energy / get_enstrophy: (1/2 interior + 1/4 lids) / ncell
         // (field_diagnostics.f90:85-206); vortrms = sqrt(get_mean(|w|^2)) (advance.f90:177)
@@ -1096,16 +1118,19 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_x(FinArgs a) {
             wm = fmax(wm, p[RM_WMAX]);
         }
     }
-    w2 = block_max(w2, red);
-    gg = block_max(gg, red);
-    us = block_max(us, red);
-    ls = block_max(ls, red);
-    um = block_max(um, red);
-    vm = block_max(vm, red);
-    wm = block_max(wm, red);
-    l1 = block_sum(l1, red);
-    l2 = block_sum(l2, red);
+    double mx[7] = {w2, gg, us, ls, um, vm, wm}, sm[2] = {l1, l2};
+    fin_reduce<7, true>(mx, red);
+    fin_reduce<2, false>(sm, red);
     if (threadIdx.x != 0) return;
+    w2 = mx[0];
+    gg = mx[1];
+    us = mx[2];
+    ls = mx[3];
+    um = mx[4];
+    vm = mx[5];
+    wm = mx[6];
+    l1 = sm[0];
+    l2 = sm[1];
     double *sc = a.scal + (size_t)blockIdx.y * SC_NSCAL;
     const double small = 1.0e-12, cflmax = 0.8;
     const double t = sc[SC_T];
