@@ -75,38 +75,57 @@ def make_params(nx, ny, nz, lower=(0.0, 0.0, 0.0), extent=(1.0, 1.0, 1.0),
 _lib = None
 
 
+def use_library(path):
+    """Load the oracle from another build of ps3d_oracle.c (the mutation tests
+    build copies with one planted mistake each) and return the library that was
+    in use, so that the caller can restore it with use_library_object()."""
+    global _lib
+    prev = _lib
+    _lib = _bind(C.CDLL(path))
+    return prev
+
+
+def use_library_object(L):
+    global _lib
+    _lib = L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB):
             raise RuntimeError("oracle not built: run `make -C oracle`")
-        L = C.CDLL(_LIB)
-        for name in ("orc_forfft", "orc_revfft", "orc_dct", "orc_dst"):
-            getattr(L, name).argtypes = [C.c_int, C.c_int, _dp]
-            getattr(L, name).restype = C.c_int
-        L.orc_jacobi_eigenvalues.argtypes = [_dp, _dp]
-        L.orc_create.argtypes = [C.POINTER(Params)]
-        L.orc_create.restype = C.c_void_p
-        L.orc_destroy.argtypes = [C.c_void_p]
-        L.orc_last_error.restype = C.c_char_p
-        L.orc_set_threads.argtypes = [C.c_int]
-        L.orc_op.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
-        L.orc_set_vorticity.argtypes = [C.c_void_p, _dp]
-        L.orc_vor2vel.argtypes = [C.c_void_p]
-        L.orc_source.argtypes = [C.c_void_p]
-        L.orc_adapt.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_void_p]
-        L.orc_step.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double]
-        L.orc_advance_n.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_void_p]
-        L.orc_derived.argtypes = [C.c_void_p]
-        L.orc_rk4_check.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
-        L.orc_get.argtypes = [C.c_void_p, C.c_int, _dp]
-        L.orc_set.argtypes = [C.c_void_p, C.c_int, _dp]
-        L.orc_table.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
-        L.orc_table.restype = C.c_long
-        L.orc_viscosity.argtypes = [C.c_void_p]
-        L.orc_viscosity.restype = C.c_double
-        _lib = L
+        _lib = _bind(C.CDLL(_LIB))
     return _lib
+
+
+def _bind(L):
+    for name in ("orc_forfft", "orc_revfft", "orc_dct", "orc_dst"):
+        getattr(L, name).argtypes = [C.c_int, C.c_int, _dp]
+        getattr(L, name).restype = C.c_int
+    L.orc_jacobi_eigenvalues.argtypes = [_dp, _dp]
+    L.orc_create.argtypes = [C.POINTER(Params)]
+    L.orc_create.restype = C.c_void_p
+    L.orc_destroy.argtypes = [C.c_void_p]
+    L.orc_last_error.restype = C.c_char_p
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_op.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+    L.orc_set_vorticity.argtypes = [C.c_void_p, _dp]
+    L.orc_vor2vel.argtypes = [C.c_void_p]
+    L.orc_source.argtypes = [C.c_void_p]
+    L.orc_adapt.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_void_p]
+    L.orc_step.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double]
+    L.orc_advance_n.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_void_p]
+    L.orc_derived.argtypes = [C.c_void_p]
+    L.orc_rk4_check.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
+    L.orc_cn2_check.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
+    L.orc_get.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_set.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.orc_table.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.orc_table.restype = C.c_long
+    L.orc_viscosity.argtypes = [C.c_void_p]
+    L.orc_viscosity.restype = C.c_double
+    return L
 
 
 def _check(st):
@@ -201,6 +220,12 @@ class Oracle:
     def rk4_check(self, dt, vdiss, src, svor):
         svor = np.ascontiguousarray(svor, dtype=np.float64).copy()
         _check(lib().orc_rk4_check(self.h, dt, np.ascontiguousarray(vdiss, dtype=np.float64),
+                                   np.ascontiguousarray(src, dtype=np.float64), svor))
+        return svor
+
+    def cn2_check(self, dt, vdiss, src, svor):
+        svor = np.ascontiguousarray(svor, dtype=np.float64).copy()
+        _check(lib().orc_cn2_check(self.h, dt, np.ascontiguousarray(vdiss, dtype=np.float64),
                                    np.ascontiguousarray(src, dtype=np.float64), svor))
         return svor
 

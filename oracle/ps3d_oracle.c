@@ -1110,25 +1110,28 @@ static void rk4_step(orc_t *o, double *t, double dt, const double *fixed_src) {
     if (!fixed_src) adjust_vorticity_mean(o);
 }
 
-/* ---- cn2 (cn2.f90:92-181) ---- */
-static void cn2_step(orc_t *o, double *t, double dt) {
+/* ---- cn2 (cn2.f90:92-181) ----
+ * fixed_src != NULL holds the tendency at fixed_src through the iterations and
+ * leaves the vorticity mean alone, as rk4_step does for test_implicit_rk. */
+static void cn2_step(orc_t *o, double *t, double dt, const double *fixed_src) {
     const size_t N = o->npts;
     const double dt2 = 0.5 * dt;
+    if (fixed_src) memcpy(o->svorts, fixed_src, sizeof(double) * 3 * N);
     for (size_t i = 0; i < 3 * N; ++i) o->vortsm[i] = o->svor[i] + dt2 * o->svorts[i];
     for (int iter = 0; iter <= 2; ++iter) {
-        if (iter > 0) { orc_vor2vel(o); vorticity_tendency(o); }
+        if (iter > 0 && !fixed_src) { orc_vor2vel(o); vorticity_tendency(o); }
         for (int nc = 0; nc < 3; ++nc) {
             double *q = o->svor + nc * N, *sm = o->vortsm + nc * N, *ts = o->svorts + nc * N;
             for (size_t i = 0; i < N; ++i) q[i] = o->filt[i] * (sm[i] + dt2 * ts[i]);
             sandwich(o, q, o->vdiss);
         }
-        adjust_vorticity_mean(o);
+        if (!fixed_src) adjust_vorticity_mean(o);
     }
     *t += dt;
 }
 
 int orc_step(orc_t *o, double *t, double dt) {
-    if (o->p.stepper == PS3D_STEPPER_CN2) cn2_step(o, t, dt);
+    if (o->p.stepper == PS3D_STEPPER_CN2) cn2_step(o, t, dt, NULL);
     else rk4_step(o, t, dt, NULL);
     return 0;
 }
@@ -1139,6 +1142,15 @@ int orc_rk4_check(orc_t *o, double dt, const double *vdiss, const double *src, d
     memcpy(o->svor, svor, sizeof(double) * 3 * o->npts);
     memcpy(o->svorts, src, sizeof(double) * 3 * o->npts);
     rk4_step(o, &t, dt, src);
+    memcpy(svor, o->svor, sizeof(double) * 3 * o->npts);
+    return 0;
+}
+
+int orc_cn2_check(orc_t *o, double dt, const double *vdiss, const double *src, double *svor) {
+    double t = 0.0;
+    memcpy(o->vdiss, vdiss, sizeof(double) * o->n2);
+    memcpy(o->svor, svor, sizeof(double) * 3 * o->npts);
+    cn2_step(o, &t, dt, src);
     memcpy(svor, o->svor, sizeof(double) * 3 * o->npts);
     return 0;
 }

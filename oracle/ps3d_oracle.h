@@ -46,6 +46,9 @@ int orc_derived(orc_t *o);
 /* impl_rk4 step with the tendency held at src (3 volumes) and vdiss (ny*nx)
  * given, svor (3 volumes) updated in place: test_implicit_rk's setting. */
 int orc_rk4_check(orc_t *o, double dt, const double *vdiss, const double *src, double *svor);
+/* cn2 step with the tendency held at src and vdiss (ny*nx) given; svor updated
+ * in place (the same setting for cn2.f90:92-181). */
+int orc_cn2_check(orc_t *o, double dt, const double *vdiss, const double *src, double *svor);
 int orc_get(orc_t *o, int which, double *out);
 int orc_set(orc_t *o, int which, const double *in);
 /* Tables for checks: 0 k2l2 (ny*nx), 1 k2l2i, 2 vhdis, 3 filt (npts), 4 green,

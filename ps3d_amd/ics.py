@@ -72,14 +72,23 @@ def beltrami_multimode(nx, ny, nz, seed=1, noise=1.0e-3, modes=BELTRAMI_MODES,
 
 def random_phase_isotropic(nx, ny, nz, k0=4.0, seed=1, lower=(0.0, 0.0, 0.0),
                            extent=(2 * PI, 2 * PI, PI), urms=1.0):
+    """Random-phase solenoidal flow with E(k) ~ k^4 exp(-2 (k/k0)^2); returns
+    the vorticity (3, nx, ny, nz+1) of random_phase_fields."""
+    return random_phase_fields(nx, ny, nz, k0, seed, lower, extent, urms)[1]
+
+
+def random_phase_fields(nx, ny, nz, k0=4.0, seed=1, lower=(0.0, 0.0, 0.0),
+                        extent=(2 * PI, 2 * PI, PI), urms=1.0):
     """Random-phase solenoidal flow with E(k) ~ k^4 exp(-2 (k/k0)^2).
 
     The velocity is the curl of a vector potential A whose x,y components are
     sine series and z component a cosine series in z, so u,v are cosine series,
     w a sine series vanishing on both lids, and div u = 0 exactly.  Each Fourier
-    mode (kx, ky, m) gets Gaussian random coefficients scaled so that the shell
-    energy follows E(k).  Returns vorticity (3, nx, ny, nz+1), velocity
-    scaled to rms speed `urms`."""
+    mode (kx, ky, m) gets complex Gaussian coefficients of rms |A| =
+    exp(-(K/k0)^2): then |u| ~ K |A| per mode, the number of modes in a shell
+    of radius k grows as k^2, and the shell energy is
+    E(k) ~ k^2 * k^2 exp(-2 (k/k0)^2), peaking at k = k0.  Returns
+    (velocity, vorticity), each (3, nx, ny, nz+1), scaled to rms speed `urms`."""
     rng = np.random.default_rng(seed)
     Lx, Ly, Lz = extent
     kx = 2 * PI * np.fft.fftfreq(nx, d=1.0 / nx) / Lx
@@ -87,9 +96,8 @@ def random_phase_isotropic(nx, ny, nz, k0=4.0, seed=1, lower=(0.0, 0.0, 0.0),
     mz = PI * np.arange(nz + 1) / Lz
     KX, KY, MZ = np.meshgrid(kx, ky, mz, indexing="ij")
     K = np.sqrt(KX ** 2 + KY ** 2 + MZ ** 2)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        # |A(k)|^2 k^2 ~ E(k)/k^2 (isotropic shell density)  ->  |A| ~ k exp(-(k/k0)^2) / k^2
-        amp = np.where(K > 0, K ** 2 * np.exp(-((K / k0) ** 2)) / np.maximum(K, 1e-300) ** 3, 0.0)
+    # per mode |u|^2 ~ K^2 |A|^2 = E(K) / K^2 (shell area)  ->  |A| = exp(-(K/k0)^2)
+    amp = np.where(K > 0, np.exp(-((K / k0) ** 2)), 0.0)
     # keep the 2/3 of the resolved band so that the field is smooth
     amp[np.abs(KX) > (2.0 / 3.0) * np.abs(kx).max()] = 0.0
     amp[np.abs(KY) > (2.0 / 3.0) * np.abs(ky).max()] = 0.0
@@ -124,4 +132,4 @@ def random_phase_isotropic(nx, ny, nz, k0=4.0, seed=1, lower=(0.0, 0.0, 0.0),
     if rms > 0:
         vel *= urms / rms
         vor *= urms / rms
-    return vor
+    return vel, vor
