@@ -139,7 +139,9 @@ int ps3d_gpu_resample(ps3d_handle *dst, ps3d_handle *src, int which);
 
 /* = vor2vel (src/inversion/inversion.f90:23-226) on the device state. */
 int ps3d_gpu_vor2vel(ps3d_handle *h);
-/* = source -> vorticity_tendency (src/inversion/inversion.f90:298-388). */
+/* = source -> vorticity_tendency (src/inversion/inversion.f90:298-388).  With a
+ * closure attached, source and step return PS3D_E_UNSUPPORTED: the closure term is
+ * formed inside the fused step of ps3d_gpu_advance_n only. */
 int ps3d_gpu_source(ps3d_handle *h);
 /* = adapt (src/stepper/advance.f90:109-377) incl. bstep%set_diffusion; needs a
  * preceding vor2vel.  Writes dt and a PS3D_NDIAG record (diag may be NULL). */
@@ -150,6 +152,14 @@ int ps3d_gpu_step(ps3d_handle *h, double *t, double dt);
 /* = advance (src/stepper/advance.f90:77-104), n times, without host round trips.
  * diag_series (may be NULL) receives n records of PS3D_NDIAG doubles. */
 int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_series);
+/* = ps3d's time loop `do while (t < time%limit); call advance(bstep, t)`
+ * (src/ps3d.f90:114-116) with the loop test on the device: up to nmax steps are
+ * queued at once (graph replays) and each runs only while t < min(t_stop, limit);
+ * the steps after the test fails return at once.  A host program stops at its
+ * output times by passing the next one as t_stop (write_step, src/utils/utils.f90:62-89).
+ * *nsteps receives the number of steps taken, diag_series (may be NULL, room for
+ * nmax records) their records.  Not with a closure attached (PS3D_E_UNSUPPORTED). */
+int ps3d_gpu_advance_until(ps3d_handle *h, double *t, double t_stop, int nmax, double *diag_series, int *nsteps);
 
 /* = pressure + horizontal_divergence (src/fields/fields_derived.f90:67-182), which
  * ps3d's adapt calls (src/stepper/advance.f90:278-282).  Computed from the state of

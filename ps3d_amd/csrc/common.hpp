@@ -177,6 +177,13 @@ struct Grid {
     double lz;                    // extent(3)
     double zlo;                   // lower(3)
     double dz;
+    // step gate: when set, every kernel of a step returns at once while gate[0] == 0
+    // (written by k_gate at the start of each step of ps3d_gpu_advance_until)
+    const double *gate;
 };
+
+#ifdef __HIPCC__
+__device__ inline bool gated_off(const double *gate) { return gate != nullptr && gate[0] == 0.0; }
+#endif
 
 }  // namespace ps3d

@@ -25,6 +25,8 @@ enum Scal {
     SC_RM_SMA,       // rolling mean state (rolling_mean.f90)
     SC_RM_INEW, SC_RM_IOLD, SC_RM_FILLED,
     SC_STEP,         // step counter (diag series row)
+    SC_GATE,         // 1 while the gated step runs (t < SC_TSTOP), else 0
+    SC_TSTOP,        // ps3d_gpu_advance_until: steps start only while t < this
     SC_NSCAL = 32
 };
 
@@ -225,9 +227,13 @@ struct FinArgs {
     int roll_len;
     double *roll_hist;
     Batch bt;
+    const double *gate;   // step gate (Grid::gate)
 };
 hipError_t launch_fin_y(const FinArgs &a, hipStream_t stream);
 hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream);
+// Opens (1) or closes (0) the step gate of a single-instance handle:
+// scal[SC_GATE] = scal[SC_T] < scal[SC_TSTOP].
+hipError_t launch_gate(double *scal, hipStream_t stream);
 
 // closure hooks (closure.hip)
 bool cnn_matches(const ps3d_cnn *c, int nx, int ny, int nz);

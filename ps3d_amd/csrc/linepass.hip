@@ -484,6 +484,7 @@ __device__ inline double eig_absmax_est(double s11, double s12, double s13, doub
 template <int N>
 __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
     extern __shared__ double2 lsmem[];
+    if (gated_off(a.g.gate)) return;
     Tile t = make_tile(a);
     if constexpr (N > 0) fix_tile(t, N, zc_for(N));
     const size_t tb = (size_t)t.nl * t.LS;
@@ -567,6 +568,7 @@ template <int N, int M, int T>
 __global__ void __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(T >= kLineThreads ? 1 : 4)))
 k_line_reg(LineArgs a) {
     extern __shared__ double2 lsmem[];
+    if (gated_off(a.g.gate)) return;
     Tile t = make_tile(a);
     if constexpr (N > 0) fix_tile(t, N, zc_reg(N, T));
     const size_t tb = (size_t)t.nl * t.LS;
@@ -1069,6 +1071,7 @@ __device__ inline void fin_reduce(double (&v)[NV], double *red) {
 
 __global__ void __launch_bounds__(kFinThreads) k_fin_y(FinArgs a) {
     __shared__ double red[kFinThreads];
+    if (gated_off(a.gate)) return;
     const double *party = a.party + (size_t)blockIdx.y * a.bt.party;
     double *scal = a.scal + (size_t)blockIdx.y * SC_NSCAL;
     double s[RY_N];
@@ -1092,6 +1095,7 @@ __global__ void __launch_bounds__(kFinThreads) k_fin_y(FinArgs a) {
 
 __global__ void __launch_bounds__(kFinThreads) k_fin_x(FinArgs a) {
     __shared__ double red[kFinThreads];
+    if (gated_off(a.gate)) return;
     double w2 = 0.0, gg = 2.220446049250313e-16, us = 0.0, ls = 0.0;
     double um = -INFINITY, vm = -INFINITY, wm = -INFINITY, l1 = 0.0, l2 = 0.0;
     const double *partx = a.partx + (size_t)blockIdx.y * a.bt.partx;
@@ -1198,6 +1202,15 @@ hipError_t launch_fin_y(const FinArgs &a, hipStream_t stream) {
 }
 hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream) {
     hipLaunchKernelGGL(k_fin_x, dim3(1, a.bt.nb > 1 ? a.bt.nb : 1), dim3(kFinThreads), 0, stream, a);
+    return hipGetLastError();
+}
+
+__global__ void k_gate(double *scal) {
+    if (threadIdx.x == 0) scal[SC_GATE] = (scal[SC_T] < scal[SC_TSTOP]) ? 1.0 : 0.0;
+}
+
+hipError_t launch_gate(double *scal, hipStream_t stream) {
+    hipLaunchKernelGGL(k_gate, dim3(1), dim3(64), 0, stream, scal);
     return hipGetLastError();
 }
 

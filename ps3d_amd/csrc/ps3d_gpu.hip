@@ -54,6 +54,12 @@ struct ps3d_handle {
     // closure, diagnostics buffer, state set-up -- changes)
     hipGraph_t sgraph = nullptr;
     hipGraphExec_t sexec = nullptr;
+    // the same step behind the device-side gate (ps3d_gpu_advance_until)
+    hipGraph_t ggraph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    // counts the times a captured argument changed (drop_step_graph): graphs that
+    // captured this handle elsewhere (an ensemble step) compare it before replaying
+    unsigned gen = 0;
     Grid g{};
     size_t nvol = 0;               // doubles per padded volume
     double *pool = nullptr;
@@ -197,10 +203,15 @@ static ZTabs ztabs(const ps3d_handle *h) {
 }
 
 static void drop_step_graph(ps3d_handle *h) {
+    ++h->gen;
     if (h->sexec) hipGraphExecDestroy(h->sexec);
     if (h->sgraph) hipGraphDestroy(h->sgraph);
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    if (h->ggraph) hipGraphDestroy(h->ggraph);
     h->sexec = nullptr;
     h->sgraph = nullptr;
+    h->gexec = nullptr;
+    h->ggraph = nullptr;
 }
 
 static void free_handle(ps3d_handle *h) {
@@ -483,6 +494,7 @@ static FinArgs fin_args(ps3d_handle *h) {
     f.roll_len = h->p.roll_mean_win_size;
     f.roll_hist = h->roll;
     f.bt = batch(h);
+    f.gate = h->g.gate;
     return f;
 }
 
@@ -947,6 +959,9 @@ int ps3d_gpu_source(ps3d_handle *h) {
     if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h) return fail(PS3D_E_ARG, "null handle");
     if (!h->have_phys) return fail(PS3D_E_STATE, "source needs a preceding vor2vel");
+    // the closure term is formed inside the fused step only (its tendency is held in
+    // the buffer this path keeps svorts in)
+    if (h->cnn) return fail(PS3D_E_UNSUPPORTED, "source with a closure attached: use ps3d_gpu_advance_n");
     int st;
     // r, q, p into grad (adapt has consumed it), then the z part into svorts (= mid)
     h->have_grad = false;
@@ -960,6 +975,7 @@ int ps3d_gpu_step(ps3d_handle *h, double *t, double dt) {
     if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !t) return fail(PS3D_E_ARG, "null argument");
     if (!h->have_adapt) return fail(PS3D_E_STATE, "step needs vor2vel, adapt and source first");
+    if (h->cnn) return fail(PS3D_E_UNSUPPORTED, "step with a closure attached: use ps3d_gpu_advance_n");
     int st;
     HIPCHK(hipMemcpyAsync(h->scal + SC_T, t, sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->scal + SC_DT, &dt, sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -986,6 +1002,56 @@ int ps3d_gpu_step(ps3d_handle *h, double *t, double dt) {
     return 0;
 }
 
+// Captures one step (with gated: the gate kernel first and every kernel of the
+// step behind it) into a graph and instantiates it.
+static int capture_step(ps3d_handle *h, bool gated, hipGraph_t *graph, hipGraphExec_t *exec) {
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
+    int st = 0;
+    if (gated) {
+        h->g.gate = h->scal + SC_GATE;
+        const hipError_t e = launch_gate(h->scal, h->stream);
+        if (e != hipSuccess) st = fail(PS3D_E_HIP, std::string("gate: ") + hipGetErrorString(e));
+    }
+    if (!st) st = one_step(h);
+    h->g.gate = nullptr;
+    hipGraph_t gr = nullptr;
+    const hipError_t ce = hipStreamEndCapture(h->stream, &gr);
+    if (st || ce != hipSuccess) {
+        if (gr) hipGraphDestroy(gr);
+        return st ? st : fail(PS3D_E_HIP, std::string("step capture: ") + hipGetErrorString(ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(exec, gr, nullptr, nullptr, 0);
+    if (ie != hipSuccess) {
+        hipGraphDestroy(gr);
+        *exec = nullptr;
+        return fail(PS3D_E_HIP, std::string("step graph: ") + hipGetErrorString(ie));
+    }
+    *graph = gr;
+    return 0;
+}
+
+// After a run of steps: time, kernel timing, the diagnostics rows (rows < 0: as
+// many as the device's row counter says were written).
+static int finish_run(ps3d_handle *h, double *t, int rows, double *diag_series) {
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms;
+    double sc[SC_NSCAL];
+    int st;
+    if ((st = read_scal(h, sc))) return st;
+    *t = sc[SC_T];
+    if (rows < 0) rows = (int)sc[SC_STEP];
+    h->last_steps = rows;
+    if (diag_series && rows > 0)
+        HIPCHK(hipMemcpy(diag_series, h->diag, sizeof(double) * PS3D_NDIAG * (size_t)rows, hipMemcpyDeviceToHost));
+    h->have_phys = false;
+    h->have_adapt = false;
+    h->have_grad = false;
+    return 0;
+}
+
 int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_series) {
     if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !t || nsteps < 0) return fail(PS3D_E_ARG, "invalid argument");
@@ -1000,22 +1066,7 @@ int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_serie
     // launches every kernel instead
     const char *gv = std::getenv("PS3D_GRAPH");
     const bool graph = nsteps >= 2 && !(gv && gv[0] == '0');
-    if (graph && !h->sexec) {
-        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
-        st = one_step(h);
-        hipGraph_t gr = nullptr;
-        const hipError_t ce = hipStreamEndCapture(h->stream, &gr);
-        if (st || ce != hipSuccess) {
-            if (gr) hipGraphDestroy(gr);
-            return st ? st : fail(PS3D_E_HIP, std::string("step capture: ") + hipGetErrorString(ce));
-        }
-        h->sgraph = gr;
-        const hipError_t ie = hipGraphInstantiate(&h->sexec, h->sgraph, nullptr, nullptr, 0);
-        if (ie != hipSuccess) {
-            drop_step_graph(h);
-            return fail(PS3D_E_HIP, std::string("step graph: ") + hipGetErrorString(ie));
-        }
-    }
+    if (graph && !h->sexec && (st = capture_step(h, false, &h->sgraph, &h->sexec))) return st;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     for (int s = 0; s < nsteps; ++s) {
         if (graph)
@@ -1023,20 +1074,25 @@ int ps3d_gpu_advance_n(ps3d_handle *h, double *t, int nsteps, double *diag_serie
         else if ((st = one_step(h)))
             return st;
     }
-    HIPCHK(hipEventRecord(h->ev1, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    h->last_ms = ms;
-    h->last_steps = nsteps;
-    double sc[SC_NSCAL];
-    if ((st = read_scal(h, sc))) return st;
-    *t = sc[SC_T];
-    if (diag_series && nsteps > 0)
-        HIPCHK(hipMemcpy(diag_series, h->diag, sizeof(double) * PS3D_NDIAG * (size_t)nsteps, hipMemcpyDeviceToHost));
-    h->have_phys = false;
-    h->have_adapt = false;
-    h->have_grad = false;
+    return finish_run(h, t, nsteps, diag_series);
+}
+
+int ps3d_gpu_advance_until(ps3d_handle *h, double *t, double t_stop, int nmax, double *diag_series, int *nsteps) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
+    if (!h || !t || nmax < 0) return fail(PS3D_E_ARG, "invalid argument");
+    if (!h->have_state) return fail(PS3D_E_STATE, "advance before set_vorticity");
+    if (h->cnn) return fail(PS3D_E_UNSUPPORTED, "advance_until with a closure attached: use ps3d_gpu_advance_n");
+    int st;
+    if ((st = ensure_diag(h, std::max(1, nmax)))) return st;
+    // SC_STEP, SC_GATE, SC_TSTOP are consecutive slots
+    const double init[3] = {0.0, 0.0, std::min(t_stop, h->p.limit)};
+    HIPCHK(hipMemcpyAsync(h->scal + SC_T, t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->scal + SC_STEP, init, sizeof init, hipMemcpyHostToDevice, h->stream));
+    if (!h->gexec && (st = capture_step(h, true, &h->ggraph, &h->gexec))) return st;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int s = 0; s < nmax; ++s) HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+    if ((st = finish_run(h, t, -1, diag_series))) return st;
+    if (nsteps) *nsteps = h->last_steps;
     return 0;
 }
 
@@ -1458,6 +1514,7 @@ struct ps3d_ensemble {
     std::vector<hipEvent_t> join;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<unsigned> gens;   // the groups' ps3d_handle::gen when the graph was captured
     double last_ms = 0.0;
 };
 
@@ -1544,6 +1601,8 @@ static int build_graph(ps3d_ensemble *e) {
     if (ce != hipSuccess) return fail(PS3D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
     e->graph = gr;
     HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    e->gens.clear();
+    for (ps3d_handle *h : e->g) e->gens.push_back(h->gen);
     return 0;
 }
 
@@ -1552,6 +1611,15 @@ int ps3d_gpu_ensemble_advance(ps3d_ensemble *e, int nsteps, int use_graph, doubl
     for (ps3d_handle *h : e->g)
         if (!h->have_state) return fail(PS3D_E_STATE, "advance before every instance has an initial vorticity");
     int st;
+    if (use_graph && e->exec) {
+        // a group's captured arguments changed (its diagnostics buffer was
+        // reallocated by advance_n_batch, say): capture again
+        for (size_t i = 0; i < e->g.size(); ++i)
+            if (e->gens[i] != e->g[i]->gen) {
+                drop_graph(e);
+                break;
+            }
+    }
     if (use_graph && !e->exec && (st = build_graph(e))) return st;
     HIPCHK(hipEventRecord(e->ev0, e->s0));
     for (int s = 0; s < nsteps; ++s) {

@@ -546,6 +546,7 @@ __device__ inline void zstore(const ZT<ZPT> &T, double *__restrict__ dst, const 
 template <int ZPT, int NZ>
 __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     const Grid g = a.g;
+    if (gated_off(g.gate)) return;
     ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads, (long)blockIdx.y * a.bt.vol);
     ZLds L = zlds(a.QPB, T.N, g.nz);
     const FftPlan &P = a.pz;
@@ -783,6 +784,7 @@ __device__ inline void rk_update(const ZT<ZPT> &T, const RkArgs &rk, int mode, i
 template <int ZPT, int NZ>
 __global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
     const Grid g = a.g;
+    if (gated_off(g.gate)) return;
     ZT<ZPT> T = make_zt<ZPT>(g, a.tb, a.TQ, a.QPB, a.nquads, (long)blockIdx.y * a.bt.vol);
     ZLds L = zlds(a.QPB, T.N, g.nz);
     const FftPlan &P = a.pz;
@@ -985,6 +987,7 @@ hipError_t launch_add2(long n, const double *a, const double *b, double *out, hi
 // the (0,0) column, plus the end-of-step time update t += dt.  One block.
 __global__ void __launch_bounds__(256) k_mean(MeanArgs a) {
     extern __shared__ double2 msmem[];
+    if (gated_off(a.g.gate)) return;
     const int n = a.g.nz, N = 2 * n, tid = threadIdx.x;
     double2 *A = msmem, *B = msmem + N;
     double *red = reinterpret_cast<double *>(B + N);

@@ -554,6 +554,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     extern __shared__ double2 wsm[];
     const Grid g = a.g;
+    if (gated_off(g.gate)) return;
     const WC c = make_wc(g, a.tb, (long)blockIdx.y * a.bt.vol);
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
     const WTw<P> tw = make_tw<P>(a.twz, c.lane);
@@ -759,6 +760,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     extern __shared__ double2 wsm[];
     const Grid g = a.g;
+    if (gated_off(g.gate)) return;
     const WC c = make_wc(g, a.tb, (long)blockIdx.y * a.bt.vol);
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
     const WTw<P> tw = make_tw<P>(a.twz, c.lane);

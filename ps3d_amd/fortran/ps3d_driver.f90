@@ -55,8 +55,13 @@ program ps3d_driver
     type(ps3d_params) :: p
     type(c_ptr) :: h
     integer(c_int) :: ncells(3), nrec, st
-    real(c_double) :: extent(3), origin(3), t, t0, tlast, dt
+    real(c_double) :: extent(3), origin(3), t, t0, tlast, dt, tstop
     real(c_double) :: row(PS3D_NDIAG)
+    ! steps queued per call of ps3d_gpu_advance_until (replays of one captured step)
+    integer(c_int), parameter :: nchunk = 32
+    real(c_double) :: rows(PS3D_NDIAG, nchunk)
+    integer(c_int) :: nd
+    integer :: k
     real(c_double), allocatable, target :: vor(:), vel(:), pres(:), delta(:)
     integer :: ios, fn, i, nfw, nsfw
     integer(c_size_t) :: nv
@@ -147,8 +152,9 @@ program ps3d_driver
         p%length_scale = PS3D_LS_KOLMOGOROV
     endif
     p%alpha = time%alpha
-    ! precise_stop: the last step ends exactly at time%limit
-    p%limit = merge(time%limit, 1.0d30, time%precise_stop)
+    ! adapt clips every step to time%limit - t (src/stepper/advance.f90:325-333);
+    ! time%precise_stop is read but not used by the step (options.f90:167)
+    p%limit = time%limit
     p%dt_fixed = time%dt_fixed
 
     nv = int(p%nx, c_size_t) * int(p%ny, c_size_t) * int(p%nz + 1, c_size_t)
@@ -175,17 +181,30 @@ program ps3d_driver
     call open_or_resume(sfile, .false.)
 
     ! --- run (src/ps3d.f90:114-126): advance = vor2vel, adapt, write_step, source, step ---
+    ! The loop test t < limit runs on the device: each call queues nchunk steps and
+    ! stops before the first one that would start at or after the next field output
+    ! time, where write_step writes the fields (t >= nfw * field_freq - eps, the
+    ! reference's t + eps >= nfw * field_freq).  Every step's record is the
+    ! field_stats row write_step would append at its time.
     do while (t < time%limit)
-        if (output%write_fields .and. t + eps >= dble(nfw) * output%field_freq) then
+        if (output%write_fields .and. t >= dble(nfw) * output%field_freq - eps) then
             call write_fields(t)
             nfw = nfw + 1
         endif
-        call gpu_check(ps3d_gpu_advance_n(h, t, 1_c_int, row), 'ps3d_gpu_advance_n')
-        if (output%write_field_stats .and. row(1) + eps >= dble(nsfw) * output%field_stats_freq) then
-            call nc_check(ps3d_nc_append_stats(cstr(sfile), row, 1_c_int), 'write_netcdf_field_stats')
-            nsfw = nsfw + 1
+        tstop = time%limit
+        if (output%write_fields) tstop = min(tstop, dble(nfw) * output%field_freq - eps)
+        call gpu_check(ps3d_gpu_advance_until(h, t, tstop, nchunk, rows, nd), 'ps3d_gpu_advance_until')
+        if (nd == 0 .and. t < tstop) then
+            write(*, '(a)') 'Error: no step taken before the stop time.'
+            error stop 1
         endif
-        write(*, '(a, es22.14, a, es22.14)') ' At time', row(1), ' and time step', row(2)
+        do k = 1, nd
+            if (output%write_field_stats .and. rows(1, k) + eps >= dble(nsfw) * output%field_stats_freq) then
+                call nc_check(ps3d_nc_append_stats(cstr(sfile), rows(:, k), 1_c_int), 'write_netcdf_field_stats')
+                nsfw = nsfw + 1
+            endif
+            write(*, '(a, es22.14, a, es22.14)') ' At time', rows(1, k), ' and time step', rows(2, k)
+        enddo
     enddo
 
     ! write_last_step (src/utils/utils.f90:53-60): forced write at the final time

@@ -68,6 +68,15 @@ module ps3d_gpu
             integer(c_int), value :: n
             real(c_double), intent(out) :: diag(*)
         end function
+        integer(c_int) function ps3d_gpu_advance_until(h, t, t_stop, nmax, diag, nsteps) bind(C)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(inout) :: t
+            real(c_double), value :: t_stop
+            integer(c_int), value :: nmax
+            real(c_double), intent(out) :: diag(*)
+            integer(c_int), intent(out) :: nsteps
+        end function
         integer(c_int) function ps3d_gpu_get(h, which, f) bind(C)
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h

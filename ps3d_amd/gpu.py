@@ -56,6 +56,8 @@ def lib():
         L.ps3d_gpu_adapt.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.c_void_p]
         L.ps3d_gpu_step.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double]
         L.ps3d_gpu_advance_n.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_void_p]
+        L.ps3d_gpu_advance_until.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int, C.c_void_p,
+                                             C.POINTER(C.c_int)]
         L.ps3d_gpu_get.argtypes = [C.c_void_p, C.c_int, _dp]
         L.ps3d_gpu_set.argtypes = [C.c_void_p, C.c_int, _dp]
         L.ps3d_gpu_op.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
@@ -185,6 +187,18 @@ class Ps3dGpu:
         _check(lib().ps3d_gpu_advance_n(self.h, C.byref(tt), int(nsteps), diag.ctypes.data))
         self.t = tt.value
         return diag[:nsteps]
+
+    def advance_until(self, t_stop, nmax):
+        """ps3d's `do while (t < limit)` loop (src/ps3d.f90:114-116) run on the
+        device up to nmax steps, stopping before the first step that would start
+        at t >= min(t_stop, limit); returns the records of the steps taken."""
+        tt = C.c_double(self.t)
+        n = C.c_int()
+        diag = np.zeros((max(1, nmax), NDIAG))
+        _check(lib().ps3d_gpu_advance_until(self.h, C.byref(tt), float(t_stop), int(nmax), diag.ctypes.data,
+                                            C.byref(n)))
+        self.t = tt.value
+        return diag[:n.value]
 
     def sync(self):
         _check(lib().ps3d_gpu_sync(self.h))

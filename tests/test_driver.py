@@ -2,8 +2,8 @@
 namelist config, field file input, time loop and NetCDF output on the library.
 
 CPU tests run the failure paths that stop before any GPU call; the GPU test runs a
-32^3 Beltrami config and compares every output record with the same run stepped
-from Python through the C ABI."""
+32^3 Beltrami config and compares every output record with ps3d's loop run on the
+oracle."""
 import os
 import subprocess
 
@@ -62,7 +62,12 @@ def test_driver_rejects_bad_namelist_and_missing_field_file(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not HAVE, reason="driver not built")
-def test_driver_run_matches_the_library_stepped_from_python(tmp_path):
+@pytest.mark.parametrize("dt_fixed,limit", [(0.01, 0.047), (0.0, 0.1)])
+def test_driver_run_matches_the_oracle(tmp_path, oracle_mod, dt_fixed, limit):
+    """The driver's field and field_stats files against ps3d's loop run on the
+    oracle (src/ps3d.f90:114-126: write_step after adapt, every step clipped to
+    time%limit - t by adapt, advance.f90:325-333; write_last_step at the end),
+    with a limit that is not a multiple of the step."""
     from scipy.io import netcdf_file
 
     from ps3d_amd import ics, ncio
@@ -74,43 +79,49 @@ def test_driver_run_matches_the_library_stepped_from_python(tmp_path):
     ic.write(0.0, np.zeros_like(vor0), vor0)
     base = str(tmp_path / "beltrami_32")
     cfg = tmp_path / "beltrami_32.config"
-    cfg.write_text(CONFIG.format(field=ic.path, base=base))
-    print("driver run", flush=True)
+    cfg.write_text(CONFIG.format(field=ic.path, base=base).replace(
+        "time%limit              = 0.05", "time%limit              = %r" % limit).replace(
+        "time%dt_fixed           = 0.01", "time%dt_fixed           = %r" % dt_fixed))
+    print("driver run dt_fixed=%g limit=%g" % (dt_fixed, limit), flush=True)
     r = run_driver(str(cfg))
     assert r.returncode == 0, r.stdout + r.stderr
 
-    p = gpu.make_params(n, n, n, **kw, filtering="2/3-rule", stepper="impl-diff-rk4", nnu=3, prediss=30.0,
-                        pretype="vorch", length_scale="Kolmogorov", alpha=0.1, dt_fixed=0.01)
-    d = gpu.Ps3dGpu(p)
-    d.set_vorticity(vor0)
-    # the driver's loop (write_step before each step, forced write at the end)
+    p = oracle_mod.make_params(n, n, n, **kw, filtering="2/3-rule", stepper="impl-diff-rk4", nnu=3, prediss=30.0,
+                               pretype="vorch", length_scale="Kolmogorov", alpha=0.1, dt_fixed=dt_fixed,
+                               limit=limit)
+    o = oracle_mod.Oracle(p)
+    o.set_vorticity(vor0)
     eps = np.finfo(float).eps
     ref, rows = [], []
     t, nfw, nsfw = 0.0, 0, 0
-    while t < 0.05:
+    while t < limit:
         if t + eps >= nfw * 0.02:
-            d.vor2vel()
-            ref.append((t, d.get(gpu.VOR), d.get(gpu.VEL)))
+            o.vor2vel()
+            ref.append((t, o.get(oracle_mod.VOR), o.get(oracle_mod.VEL)))
             nfw += 1
-        row = d.advance_n(1)[0]
-        t = d.t
+        row = o.advance_n(1)[0]
+        t = o.t
         if row[0] + eps >= nsfw * 0.01:
             rows.append(row)
             nsfw += 1
-    d.vor2vel()
-    ref.append((t, d.get(gpu.VOR), d.get(gpu.VEL)))
+    o.vor2vel()
+    ref.append((t, o.get(oracle_mod.VOR), o.get(oracle_mod.VEL)))
+    rows.append(o.adapt(t)[1])
+    o.close()
+    assert abs(t - limit) < 1e-15
     with netcdf_file(base + "_fields.nc", "r", mmap=False) as f:
         times = np.array(f.variables["t"].data)
-        np.testing.assert_array_equal(times, [r[0] for r in ref])
+        np.testing.assert_allclose(times, [r[0] for r in ref], rtol=0, atol=1e-15)
         for k, (tk, vor, vel) in enumerate(ref):
             for c, x in enumerate("xyz"):
-                np.testing.assert_allclose(f.variables[x + "_vorticity"][k], vor[c].T, rtol=0, atol=1e-12)
-                np.testing.assert_allclose(f.variables[x + "_velocity"][k], vel[c].T, rtol=0, atol=1e-12)
+                np.testing.assert_allclose(f.variables[x + "_vorticity"][k], vor[c].T, rtol=0,
+                                           atol=1e-12 * np.abs(vor[c]).max())
+                np.testing.assert_allclose(f.variables[x + "_velocity"][k], vel[c].T, rtol=0,
+                                           atol=1e-12 * np.abs(vel[c]).max())
+    rows = np.array(rows)
     with netcdf_file(base + "_field_stats.nc", "r", mmap=False) as f:
         tt = np.array(f.variables["t"].data)
-        ke = np.array(f.variables["ke"].data)
-        assert len(ke) == len(rows) + 1              # the steps' rows, then the final time
-        np.testing.assert_array_equal(tt[:-1], np.array(rows)[:, 0])
-        np.testing.assert_allclose(ke[:-1], np.array(rows)[:, 2], rtol=1e-13)
-        assert tt[-1] == t
-    d.close()
+        assert len(tt) == len(rows)
+        np.testing.assert_allclose(tt, rows[:, 0], rtol=0, atol=1e-15)
+        for name, col in (("ke", 2), ("en", 3)):
+            np.testing.assert_allclose(np.array(f.variables[name].data), rows[:, col], rtol=1e-12)

@@ -52,9 +52,16 @@ def diag_rel(da, db, cols):
     return out
 
 
-# diagnostics compared: all but the vorticity means (O(1e-5) noise means whose
-# relative error is dominated by roundoff of O(1) sums)
+# diagnostics compared relatively: all but the vorticity means (O(1e-5) noise
+# means whose relative error is dominated by roundoff of O(1) sums), which
+# vormean_abs compares absolutely, on the scale of the rms vorticity
 DIAG_COLS = [0, 1, 2, 3, 4, 5, 6, 7, 11, 12, 13, 14, 15, 16, 17, 18]
+
+
+def vormean_abs(da, db):
+    """Largest difference of the vorticity means (columns 8-10) over the series,
+    in units of the largest rms vorticity (column 6)."""
+    return np.abs(da[:, 8:11] - db[:, 8:11]).max() / np.abs(db[:, 6]).max()
 
 
 def test_setup_matches_oracle(oracle_mod, G):
@@ -108,14 +115,72 @@ def test_beltrami32_fixed_dt_trajectory(oracle_mod, G, nsteps):
     print("gpu run", flush=True)
     dd = d.advance_n(nsteps)
     r = diag_rel(dd, do, DIAG_COLS)
-    print("diag rel", r, flush=True)
+    print("diag rel", r, "vormean", vormean_abs(dd, do), flush=True)
     assert max(r.values()) < TOL, r
+    assert vormean_abs(dd, do) < TOL
     o.vor2vel()
     d.vor2vel()
     for which in (G.SVOR, G.VEL, G.VOR):
         fr = field_rel(d.get(which), o.get(which))
         print("field", which, fr, flush=True)
         assert fr < TOL, which
+    d.close()
+
+
+def test_advance_until_stops_like_the_time_loop(oracle_mod, G):
+    """ps3d_gpu_advance_until runs ps3d's `do while (t < limit)` loop on the
+    device: steps start only while t < min(t_stop, limit), the remaining queued
+    steps do nothing, the last step is clipped to the limit; the state equals
+    advance_n's bit for bit and the oracle's to the trajectory bar."""
+    cfg = config1(limit=0.047)
+    o, d = make_pair(oracle_mod, G, 32, **cfg)
+    _, e = make_pair(oracle_mod, G, 32, **cfg)
+    vor = ics.beltrami_multimode(32, 32, 32, seed=1)
+    for x in (o, d, e):
+        x.set_vorticity(vor)
+    print("advance_until to 0.025", flush=True)
+    r1 = d.advance_until(0.025, 8)          # steps at t = 0, 0.01, 0.02
+    assert len(r1) == 3 and abs(d.t - 0.03) < 1e-15
+    ref = e.advance_n(3)
+    assert np.array_equal(r1, ref) and np.array_equal(d.get(G.SVOR), e.get(G.SVOR))
+    print("advance_until to the limit", flush=True)
+    r2 = d.advance_until(1.0e30, 8)         # t = 0.03, 0.04 (dt 0.007) -> 0.047
+    assert len(r2) == 2 and abs(d.t - 0.047) < 1e-15, (len(r2), d.t)
+    assert r2[-1, 1] == pytest.approx(0.007, abs=1e-15)
+    assert len(d.advance_until(1.0e30, 4)) == 0 and d.t == pytest.approx(0.047, abs=1e-15)
+    rows = np.concatenate([r1, r2])
+    do = np.concatenate([o.advance_n(1) for _ in range(5)])
+    assert max(diag_rel(rows, do, DIAG_COLS).values()) < TOL
+    assert vormean_abs(rows, do) < TOL
+    assert field_rel(d.get(G.SVOR), o.get(oracle_mod.SVOR)) < TOL
+    for x in (o, d, e):
+        x.close()
+
+
+def test_closure_refuses_the_one_call_path(G):
+    """With a closure attached the closure term exists only inside the fused
+    step: source and step refuse (PS3D_E_UNSUPPORTED = 6), advance_n runs."""
+    from ps3d_amd import closure
+    p = G.make_params(16, 16, 16, **{k: v for k, v in box(16).items() if k in ("lower", "extent")}, **config1())
+    d = G.Ps3dGpu(p)
+    d.set_vorticity(ics.beltrami_multimode(16, 16, 16, seed=3))
+    net = closure.ClosureNet(16, 16, 16)
+    net.set_weights(closure.init_weights(1))
+    net.attach(d)
+    d.vor2vel()
+    dt, _ = d.adapt(0.0)
+    with pytest.raises(G.Ps3dError) as ei:
+        d.source()
+    assert ei.value.code == 6
+    with pytest.raises(G.Ps3dError) as ei:
+        d.advance_until(1.0, 2)
+    assert ei.value.code == 6
+    d.advance_n(2)
+    closure.ClosureNet.detach(d)
+    d.vor2vel()
+    d.adapt(d.t)
+    d.source()
+    net.close()
     d.close()
 
 
