@@ -24,7 +24,7 @@ sys.path.insert(0, HERE)
 
 PEAK_HBM = 8.0e12          # MI355X HBM3E, B/s
 B_STEP_VOLUMES = 218       # algorithmic full-volume FP64 touches per RK4 step (fused floor)
-KIND_NAMES = ["zinv", "yinv", "xmain", "yfwd", "ztend", "yinv_adapt", "xadapt", "reduce", "mean"]
+KIND_NAMES = ["zinv", "yinv", "xmain", "yfwd", "ztend", "yinv_adapt", "xadapt", "reduce", "mean", "ztend_zinv"]
 
 
 def parse():
@@ -35,7 +35,8 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="grid size n^3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: the process's CPU share (OMP_NUM_THREADS if set, else all usable CPUs)")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--ic", choices=["beltrami", "random"], default="beltrami",
                     help="beltrami: 4 seeded Beltrami modes + noise; random: random-phase isotropic "
@@ -155,9 +156,42 @@ def initial_vorticity(G, ics, n, kind):
     return None, coarse
 
 
+def host_cpu():
+    """Model name, logical CPUs usable by this process and physical cores of the host."""
+    model, cores = None, set()
+    phys = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    cores.add((phys, v))
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": usable, "logical_cpus": os.cpu_count(),
+            "physical_cores": len(cores) or None}
+
+
 def cpu_baseline(vor, n, steps, threads):
+    """The oracle (C FP64 restatement of ps3d's unfused step) on the host cores: the
+    same workload on all usable cores, plus one core on the same workload at a
+    quarter of the linear size (a bounded sample), scaled to the full grid by the
+    operation count ratio N log N."""
+    import math
     from oracle import oracle as O
-    ncores = threads if threads > 0 else (os.cpu_count() or 1)
+    host = host_cpu()
+    # the process's CPU share: OMP_NUM_THREADS where the host sets one, else every usable CPU
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or host["nproc"]
+    ncores = threads if threads > 0 else min(share, host["nproc"])
     O.set_threads(ncores)
     o = O.Oracle(workload_params(O, n))
     o.set_vorticity(vor)
@@ -165,9 +199,28 @@ def cpu_baseline(vor, n, steps, threads):
     o.advance_n(steps)
     el = time.perf_counter() - t0
     o.close()
-    return {"value": steps / el, "unit": "steps/s", "cores": ncores, "kind": "port",
-            "sample": "%d step(s) of the same %d^3 workload through the C FP64 restatement of ps3d's "
-                      "unfused step (oracle/ps3d_oracle.c), OpenMP over %d host threads" % (steps, n, ncores)}
+    out = {"value": steps / el, "unit": "steps/s", "cores": ncores, "kind": "port",
+           "sample": "%d step(s) of the same %d^3 workload through the C FP64 restatement of ps3d's "
+                     "unfused step (oracle/ps3d_oracle.c), OpenMP over %d host threads" % (steps, n, ncores),
+           "host": host}
+    # one core: n/2 grid (one eighth of the points), 1 step
+    from ps3d_amd import ics
+    ns = max(32, n // 2)
+    O.set_threads(1)
+    o = O.Oracle(workload_params(O, ns))
+    o.set_vorticity(ics.beltrami_multimode(ns, ns, ns, seed=1))
+    t0 = time.perf_counter()
+    o.advance_n(1)
+    el1 = time.perf_counter() - t0
+    o.close()
+    O.set_threads(ncores)
+    npts, nps = n * n * (n + 1), ns * ns * (ns + 1)
+    ratio = (npts * math.log(npts)) / (nps * math.log(nps))
+    out["one_core"] = {"value": 1.0 / (el1 * ratio), "unit": "steps/s", "cores": 1, "kind": "port",
+                       "measured": {"grid": ns, "steps": 1, "seconds": el1},
+                       "sample": "1 step of the same workload at %d^3 on one host thread, scaled to %d^3 by "
+                                 "the operation-count ratio N log N = %.3f" % (ns, n, ratio)}
+    return out
 
 
 def main():
@@ -206,9 +259,9 @@ def main():
     # per-kernel durations (HIP events on the handle's stream), outside the timed region
     import ctypes as C
     import numpy as np
-    ms = np.zeros(9)
-    launches = np.zeros(9, dtype=np.int32)
-    nbytes = np.zeros(9)
+    ms = np.zeros(len(KIND_NAMES))
+    launches = np.zeros(len(KIND_NAMES), dtype=np.int32)
+    nbytes = np.zeros(len(KIND_NAMES))
     st = G.lib().ps3d_gpu_profile_kernels(d.h, a.profile_steps, ms.ctypes.data_as(C.c_void_p),
                                           launches.ctypes.data_as(C.c_void_p), nbytes.ctypes.data_as(C.c_void_p))
     kernels = {}
@@ -245,6 +298,9 @@ def main():
         "config": {"workload": "plain FP64 %s %d^3, impl-diff-rk4, 2/3-rule, adaptive CFL dt"
                                % ("Beltrami" if a.ic == "beltrami" else "random-phase isotropic", n),
                    "grid": [n, n, n], "npts": npts,
+                   "pressure_divergence": "output steps only (ps3d_gpu_derived); ps3d computes them in every "
+                                          "adapt (src/stepper/advance.f90:278-282) although they never feed "
+                                          "the step, and the algorithmic byte count leaves them out",
                    "ic": ("4 Beltrami modes, seeded phases, 1e-3 noise (seed 1)" if a.ic == "beltrami" else
                           "random-phase isotropic E(k)~k^4 exp(-2(k/4)^2), seed 1, made on 128^3, zero-padded")},
         "grid_point_updates_per_s": npts * steps_per_s,

@@ -93,6 +93,20 @@ enum ps3d_diag {
     PS3D_NDIAG
 };
 
+/* ps3d's field_stats record, in the order of its variables NC_KE .. NC_ROMAX
+ * (src/fields/field_diagnostics_netcdf.f90:36-75); PS3D_NSTAT doubles. */
+enum ps3d_stat {
+    PS3D_ST_KE = 0, PS3D_ST_EN, PS3D_ST_OMAX, PS3D_ST_ORMS, PS3D_ST_OCHAR,
+    PS3D_ST_OXMEAN, PS3D_ST_OYMEAN, PS3D_ST_OZMEAN,
+    PS3D_ST_KEXY, PS3D_ST_KEZ, PS3D_ST_ENXY, PS3D_ST_ENZ,
+    PS3D_ST_OXMIN, PS3D_ST_OYMIN, PS3D_ST_OZMIN, PS3D_ST_OXMAX, PS3D_ST_OYMAX, PS3D_ST_OZMAX,
+    PS3D_ST_HEMAX, PS3D_ST_GMAX, PS3D_ST_BFMAX, PS3D_ST_UMAX, PS3D_ST_VMAX, PS3D_ST_WMAX,
+    PS3D_ST_USOXMAX, PS3D_ST_LSOXMAX, PS3D_ST_USOYMAX, PS3D_ST_LSOYMAX, PS3D_ST_USOZMAX, PS3D_ST_LSOZMAX,
+    PS3D_ST_USUHMAX, PS3D_ST_USGMAX, PS3D_ST_LSGMAX, PS3D_ST_USZRMS, PS3D_ST_USDELRMS,
+    PS3D_ST_RGMAX, PS3D_ST_RBFMAX, PS3D_ST_RIMIN, PS3D_ST_ROMIN, PS3D_ST_ROMAX,
+    PS3D_NSTAT
+};
+
 /* Single operators, for kernel checks (ps3d_gpu_op). Each reads `in` and writes
  * `out` (one volume each, host layout) on the handle's grid. */
 enum ps3d_op {
@@ -169,6 +183,20 @@ int ps3d_gpu_advance_until(ps3d_handle *h, double *t, double t_stop, int nmax, d
  * PS3D_PRES / PS3D_DELTA. */
 int ps3d_gpu_derived(ps3d_handle *h);
 
+/* = update_netcdf_field_diagnostics (src/fields/field_diagnostics_netcdf.f90:257-439)
+ * together with the values adapt hands to set_netcdf_field_diagnostic
+ * (src/stepper/advance.f90:188-193, 315-321, 366): the PS3D_NSTAT record
+ * (enum ps3d_stat) of the state of the last ps3d_gpu_vor2vel, written to the host
+ * array stats.  diag is the PS3D_NDIAG adapt record of the same time (from
+ * ps3d_gpu_adapt, or that step's row of ps3d_gpu_advance_n / _until); NULL leaves
+ * the adapt entries 0.  The sums, minima and maxima are reductions over the
+ * device's physical velocity and vorticity; usdeltarms uses the horizontal
+ * divergence, which this call computes with ps3d_gpu_derived unless that already
+ * ran after the vor2vel.  ro_min / ro_max divide by f_cor(3) as ps3d does (inf
+ * or nan when it is 0); bfmax, rolling_mean_bfmax and ri_min are 0 (no buoyancy).
+ * The partials and the device copy of the record are allocated on first use. */
+int ps3d_gpu_field_stats(ps3d_handle *h, const double *diag, double *stats);
+
 /* Hyperdiffusion coefficients: vvisc of init_diffusion (inversion_utils.f90:157-184)
  * and the prefactor of the last adapt (get_diffusion_pre_factor, advance.f90:381-410);
  * column (k, l) decays at the rate pref * visc * (k^2 + l^2)^nnu.  Either pointer may
@@ -207,7 +235,8 @@ enum ps3d_kernel_kind {
     PS3D_K_XADAPT = 6,    /* strain / CFL / vorticity diagnostics                */
     PS3D_K_REDUCE = 7,    /* one-block reductions                                */
     PS3D_K_MEAN = 8,      /* vorticity-mean restore + time update                */
-    PS3D_NKIND = 9
+    PS3D_K_ZTEND_ZINV = 9,/* RK4 stage 1-3 z pass + the next stage's z inverse   */
+    PS3D_NKIND = 10
 };
 
 /* Runs nsteps of advance with a HIP event pair around every kernel on the

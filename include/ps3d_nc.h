@@ -1,5 +1,7 @@
-/* ps3d_nc.h -- ps3d's field and field_stats files (NetCDF classic, 64-bit offset
- * format), written and read by host code without libnetcdf.
+/* ps3d_nc.h -- ps3d's field and field_stats files in the NetCDF classic data model,
+ * written and read by host code without libnetcdf: the 64-bit offset format
+ * (CDF-2) where a variable's record fits 4 GiB, else the 64-bit data format (CDF-5,
+ * e.g. a 1024^3 field record of 8.6 GB).
  *
  * Field files carry dimensions t (unlimited), z (nz+1), y, x, coordinates x, y, z
  * and record variables t, x/y/z_velocity, x/y/z_vorticity and optionally pressure
@@ -8,12 +10,17 @@
  * vel/vor contiguous one after another (3 volumes).
  *
  * ps3d writes NetCDF-4 files with a physical_quantities group; the classic format
- * has no groups, so those quantities are not written.  Fields larger than 4 GiB per
- * variable and record (beyond the 64-bit offset format's limit) are rejected.
+ * has no groups, so those quantities are not written.
  * Every entry returns 0 or a ps3d_status code; ps3d_nc_last_error() explains.
  */
 #ifndef PS3D_NC_H
 #define PS3D_NC_H
+
+#include "ps3d_gpu.h"
+
+/* One field_stats record: t, the PS3D_NSTAT statistics (enum ps3d_stat), then the
+ * extensions helicity, dt and prefactor. */
+#define PS3D_NC_STATS_ROW (PS3D_NSTAT + 4)
 
 #ifdef __cplusplus
 extern "C" {
@@ -21,12 +28,18 @@ extern "C" {
 
 const char *ps3d_nc_last_error(void);
 
+/* Flags of ps3d_nc_create_fields. */
+enum ps3d_nc_flags {
+    PS3D_NC_DERIVED = 1,   /* add pressure and horizontal_divergence                 */
+    PS3D_NC_CDF5 = 2       /* 64-bit data format even when every record fits CDF-2   */
+};
+
 /* = create_netcdf_field_file (src/fields/field_netcdf.f90:65-139) with the axes of
  * write_netcdf_axis_3d (src/netcdf/netcdf_writer.f90:390-432): creates (or
- * truncates) `path`, writes the x, y, z coordinates (lower + i*extent/n).
- * with_derived != 0 adds pressure and horizontal_divergence. */
+ * truncates) `path`, writes the x, y, z coordinates (lower + i*extent/n).  The
+ * format is CDF-2 unless a variable's record reaches 4 GiB (then CDF-5). */
 int ps3d_nc_create_fields(const char *path, int nx, int ny, int nz, const double *lower,
-                          const double *extent, int with_derived);
+                          const double *extent, int flags);
 
 /* = write_netcdf_fields (field_netcdf.f90:177-254): appends one record at time t.
  * vel, vor: 3 volumes each; pres, delta: 1 volume each, required when the file was
@@ -53,13 +66,21 @@ int ps3d_nc_grid(const char *path, int *ncells, double *extent, double *origin);
 int ps3d_nc_records(const char *path, int *nrecords, double *t_last);
 
 /* = create_netcdf_field_stats_file (src/fields/field_diagnostics_netcdf.f90:124-188):
- * one record variable per entry of the diagnostics record (enum ps3d_diag), named
- * as ps3d names the quantity where it has one (ke, en, vortmax, vortrms, vorch,
- * x/y/z_vormean, gmax, umax, vmax, wmax, usgmax, lsgmax, rolling_mean_gmax). */
-int ps3d_nc_create_stats(const char *path);
+ * the record variable t and ps3d's 40 statistics in ps3d's order with its names,
+ * long names and units (36-75, 453-798), then the extension variables helicity,
+ * dt and prefactor; global attributes file_type, ncells, extent and origin
+ * (write_netcdf_box, src/netcdf/netcdf_writer.f90:355-369). */
+int ps3d_nc_create_stats(const char *path, int nx, int ny, int nz, const double *lower, const double *extent);
+
+/* Assembles a PS3D_NC_STATS_ROW record from the adapt record diag (PS3D_NDIAG, from
+ * ps3d_gpu_adapt or a step's row of ps3d_gpu_advance_n / _until) and the field
+ * statistics of the same state (PS3D_NSTAT, ps3d_gpu_field_stats): t from diag, the
+ * entries ps3d's adapt sets (vortmax .. z_vormean, gmax, umax, vmax, wmax, usgmax,
+ * lsgmax, rolling_mean_gmax) from diag, the others from stats. */
+int ps3d_nc_stats_row(const double *diag, const double *stats, double *row);
 
 /* = write_netcdf_field_stats (field_diagnostics_netcdf.f90:211-252): appends nrows
- * diagnostics records (PS3D_NDIAG doubles each, as ps3d_gpu_advance_n returns). */
+ * records of PS3D_NC_STATS_ROW doubles. */
 int ps3d_nc_append_stats(const char *path, const double *rows, int nrows);
 
 #ifdef __cplusplus

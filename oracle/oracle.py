@@ -82,6 +82,8 @@ def use_library(path):
     global _lib
     prev = _lib
     _lib = _bind(C.CDLL(path))
+    if _threads is not None:
+        _lib.orc_set_threads(_threads)
     return prev
 
 
@@ -117,6 +119,7 @@ def _bind(L):
     L.orc_step.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double]
     L.orc_advance_n.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_void_p]
     L.orc_derived.argtypes = [C.c_void_p]
+    L.orc_field_stats.argtypes = [C.c_void_p, C.c_void_p, _dp]
     L.orc_rk4_check.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
     L.orc_cn2_check.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp]
     L.orc_get.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -133,8 +136,13 @@ def _check(st):
         raise RuntimeError("oracle error %d: %s" % (st, lib().orc_last_error().decode()))
 
 
+_threads = None   # the thread count last set, applied to libraries loaded later
+
+
 def set_threads(n):
-    lib().orc_set_threads(int(n))
+    global _threads
+    _threads = int(n)
+    lib().orc_set_threads(_threads)
 
 
 def fft1d(kind, x):
@@ -216,6 +224,13 @@ class Oracle:
 
     def derived(self):
         _check(lib().orc_derived(self.h))
+
+    def field_stats(self, diag=None):
+        """The 40 field_stats values (NC_KE .. NC_ROMAX) after vor2vel and derived."""
+        out = np.zeros(40)
+        d = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
+        _check(lib().orc_field_stats(self.h, None if d is None else d.ctypes.data, out))
+        return out
 
     def rk4_check(self, dt, vdiss, src, svor):
         svor = np.ascontiguousarray(svor, dtype=np.float64).copy()

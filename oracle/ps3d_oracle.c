@@ -751,6 +751,90 @@ static double get_helicity(orc_t *o) {
         }
     return (0.5 * sl + si) / (double)o->ncell;
 }
+/* Trapezoid sums (1/2 inside, 1/4 on the lids) of f_c^2 over components c0..c1
+ * of a 3-volume field: get_horizontal/vertical_kinetic_energy and
+ * get_horizontal/vertical_enstrophy (field_diagnostics.f90:128-262). */
+static double trapezoid_sq(orc_t *o, const double *vv, int c0, int c1) {
+    const int nz = o->nz;
+    double t1 = 0, t0 = 0, tn = 0;
+    for (size_t col = 0; col < o->n2; ++col) {
+        double a1 = 0, a0 = 0, an = 0;
+        for (int c = c0; c <= c1; ++c) {
+            const double *f = vv + (size_t)c * o->npts + col * o->nz1;
+            for (int iz = 1; iz <= nz - 1; ++iz) a1 += f[iz] * f[iz];
+            a0 += f[0] * f[0];
+            an += f[nz] * f[nz];
+        }
+        t1 += a1; t0 += a0; tn += an;
+    }
+    return (0.5 * t1 + 0.25 * t0 + 0.25 * tn) * o->ncelli;
+}
+
+/* update_netcdf_field_diagnostics (field_diagnostics_netcdf.f90:257-439) plus the
+ * values adapt hands over (advance.f90:188-193, 315-321, 366): the 40 field_stats
+ * variables in the order NC_KE .. NC_ROMAX (36-75).  Needs orc_vor2vel and
+ * orc_derived of the same state; diag is the adapt record (may be NULL). */
+int orc_field_stats(orc_t *o, const double *diag, double *st) {
+    const int nz = o->nz;
+    const size_t N = o->npts;
+    const double *xi = o->vor, *eta = o->vor + N, *zeta = o->vor + 2 * N;
+    const double *u = o->vel, *v = o->vel + N;
+    for (int k = 0; k < 40; ++k) st[k] = 0.0;
+    st[0] = get_kinetic_energy(o, o->vel);                /* NC_KE  */
+    st[1] = get_kinetic_energy(o, o->vor);                /* NC_EN: get_enstrophy has the same weights */
+    if (diag) {
+        st[2] = diag[PS3D_D_VORTMAX];
+        st[3] = diag[PS3D_D_VORTRMS];
+        st[4] = diag[PS3D_D_VORCH];
+        st[5] = diag[PS3D_D_VORMEAN_X];
+        st[6] = diag[PS3D_D_VORMEAN_Y];
+        st[7] = diag[PS3D_D_VORMEAN_Z];
+        st[19] = diag[PS3D_D_GGMAX];
+        st[21] = diag[PS3D_D_UMAX];
+        st[22] = diag[PS3D_D_VMAX];
+        st[23] = diag[PS3D_D_WMAX];
+        st[31] = diag[PS3D_D_USGGMAX];
+        st[32] = diag[PS3D_D_LSGGMAX];
+        st[35] = diag[PS3D_D_RGMAX];
+    }
+    st[8] = trapezoid_sq(o, o->vel, 0, 1);                /* kexy */
+    st[9] = trapezoid_sq(o, o->vel, 2, 2);                /* kez  */
+    st[10] = trapezoid_sq(o, o->vor, 0, 1);               /* enxy */
+    st[11] = trapezoid_sq(o, o->vor, 2, 2);               /* enz  */
+    double mn[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, mx[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL}, hemax = -HUGE_VAL;
+    for (size_t i = 0; i < N; ++i) {
+        const double w3[3] = {xi[i], eta[i], zeta[i]};
+        for (int c = 0; c < 3; ++c) {
+            if (w3[c] < mn[c]) mn[c] = w3[c];
+            if (w3[c] > mx[c]) mx[c] = w3[c];
+        }
+        const double he = sqrt(xi[i] * xi[i] + eta[i] * eta[i]);   /* get_max_horizontal_enstrophy */
+        if (he > hemax) hemax = he;
+    }
+    for (int c = 0; c < 3; ++c) { st[12 + c] = mn[c]; st[15 + c] = mx[c]; }
+    st[18] = hemax;
+    /* lids: maxval(vor(nz,:,:,c)), maxval(vor(0,:,:,c)), sqrt(maxval(u^2 + v^2)) at nz */
+    double lid[6] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL}, uh = -HUGE_VAL;
+    double sz = 0, sd = 0;
+    for (size_t col = 0; col < o->n2; ++col) {
+        const size_t t = col * o->nz1 + nz, b = col * o->nz1;
+        const double *w3[3] = {xi, eta, zeta};
+        for (int c = 0; c < 3; ++c) {
+            lid[2 * c] = fmax(lid[2 * c], w3[c][t]);
+            lid[2 * c + 1] = fmax(lid[2 * c + 1], w3[c][b]);
+        }
+        uh = fmax(uh, u[t] * u[t] + v[t] * v[t]);
+        sz += zeta[t] * zeta[t];
+        sd += o->delta[t] * o->delta[t];
+    }
+    for (int k = 0; k < 6; ++k) st[24 + k] = lid[k];      /* uoxmax loxmax uoymax loymax uozmax lozmax */
+    st[30] = sqrt(uh);                                    /* usuhmax */
+    st[33] = sqrt(sz / (double)(o->nx * o->ny));          /* uszrms */
+    st[34] = sqrt(sd / (double)(o->nx * o->ny));          /* usdeltarms */
+    st[38] = mn[2] / o->p.f_cor[2];                       /* ro_min (field_diagnostics.f90:293-304) */
+    st[39] = mx[2] / o->p.f_cor[2];                       /* ro_max (309-320) */
+    return 0;
+}
 /* calc_vorticity_mean: 584-599 */
 static void calc_vorticity_mean(orc_t *o, double *savg) {
     const int nz = o->nz;

@@ -43,6 +43,8 @@ int orc_adapt(orc_t *o, double t, double *dt, double *diag);
 int orc_step(orc_t *o, double *t, double dt);
 int orc_advance_n(orc_t *o, double *t, int nsteps, double *diag_series);
 int orc_derived(orc_t *o);
+/* field_stats record (40 values, NC_KE .. NC_ROMAX) after vor2vel + derived */
+int orc_field_stats(orc_t *o, const double *diag, double *stats);
 /* impl_rk4 step with the tendency held at src (3 volumes) and vdiss (ny*nx)
  * given, svor (3 volumes) updated in place: test_implicit_rk's setting. */
 int orc_rk4_check(orc_t *o, double dt, const double *vdiss, const double *src, double *svor);

@@ -15,7 +15,7 @@ BUILD = os.path.join(ROOT, "_build")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libps3d_gpu.so")
 
-SOURCES = ["zpass.hip", "zwave.hip", "linepass.hip", "closure.hip", "ps3d_gpu.hip", "ncfile.cpp"]
+SOURCES = ["zpass.hip", "zwave.hip", "linepass.hip", "closure.hip", "stats.hip", "ps3d_gpu.hip", "ncfile.cpp"]
 HEADERS = ["common.hpp", "kernels.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

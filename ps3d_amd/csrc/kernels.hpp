@@ -135,6 +135,10 @@ hipError_t configure_zwave();
 hipError_t launch_zinv_w(int nz, int variant, int nquads, const ZInvArgs &args, hipStream_t stream);
 hipError_t launch_ztend_w(int nz, int variant, int nquads, const ZTendArgs &args, hipStream_t stream);
 hipError_t launch_zop_w(int nz, int nquads, const ZOpArgs &args, hipStream_t stream);
+// RK4 stage 1..3 of the tendency pass fused with the next stage's vor2vel pass (the
+// stage result stays in registers; t.rk.q is not written)
+hipError_t launch_ztend_zinv_w(int nz, int variant, int nquads, const ZTendArgs &t, const ZInvArgs &z,
+                               hipStream_t stream);
 hipError_t launch_mean(const MeanArgs &args, int nz, hipStream_t stream);
 
 // Pointwise kernels of the output-step derived fields (fields_derived.f90:67-182),
@@ -234,6 +238,37 @@ hipError_t launch_fin_x(const FinArgs &a, hipStream_t stream);
 // Opens (1) or closes (0) the step gate of a single-instance handle:
 // scal[SC_GATE] = scal[SC_T] < scal[SC_TSTOP].
 hipError_t launch_gate(double *scal, hipStream_t stream);
+
+// ---------------------------------------------------------------------------
+// output-step reductions (stats.hip)
+// ---------------------------------------------------------------------------
+// Per-block partial slots of the field_stats reduction: sums, minima, maxima.
+enum {
+    ST_S0 = 0, ST_S_UV = 0, ST_S_W, ST_S_XE, ST_S_Z, ST_S_ZTOP, ST_S_DTOP,
+    ST_MIN0, ST_MIN_X = ST_MIN0, ST_MIN_Y, ST_MIN_Z,
+    ST_MAX0, ST_MAX_X = ST_MAX0, ST_MAX_Y, ST_MAX_Z, ST_MAX_HE2, ST_MAX_UOX, ST_MAX_UOY, ST_MAX_UOZ,
+    ST_MAX_USUH2, ST_MAX_LOX, ST_MAX_LOY, ST_MAX_LOZ,
+    ST_NP,
+    ST_NSUM = ST_MIN0 - ST_S0, ST_NMIN = ST_MAX0 - ST_MIN0, ST_NMAX = ST_NP - ST_MAX0
+};
+constexpr int kStatsMaxBlocks = 1024;   // blocks of the partials pass (any grid)
+
+struct StatsArgs {
+    int nx, ny, nz, zs;
+    const double *vel[3], *vor[3];
+    const double *delta;        // physical horizontal divergence, or nullptr (usdeltarms = 0)
+    double *part;               // nb * ST_NP partials
+    int nb;
+};
+struct StatsFin {
+    const double *part;
+    int nb;
+    double ncelli, nxnyi, fcor3;
+    double ad[32];              // the adapt record of the same time (enum ps3d_diag)
+    double *out;                // enum ps3d_stat record
+};
+int stats_blocks(int nx, int ny);
+hipError_t launch_stats(const StatsArgs &a, const StatsFin &f, hipStream_t stream);
 
 // closure hooks (closure.hip)
 bool cnn_matches(const ps3d_cnn *c, int nx, int ny, int nz);

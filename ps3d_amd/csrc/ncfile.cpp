@@ -1,5 +1,7 @@
-// ncfile.cpp -- ps3d's field and field_stats files in NetCDF classic format
-// (64-bit offset variant, "CDF-2"), written and read without libnetcdf.
+// ncfile.cpp -- ps3d's field and field_stats files in NetCDF classic format,
+// written and read without libnetcdf: the 64-bit offset variant ("CDF-2") where
+// every variable's record fits its 32-bit size field, else the 64-bit data
+// variant ("CDF-5", the format ps3d's 1024^3 records need).
 //
 // Field file (src/fields/field_netcdf.f90:65-272, 507-575): dimensions t (unlimited), z
 // (nz+1), y, x; coordinates x, y, z; record variables t and x/y/z_velocity,
@@ -11,12 +13,14 @@
 // so those quantities are not written.
 //
 // field_stats file (src/fields/field_diagnostics_netcdf.f90:124-252): dimension t
-// (unlimited) and one record variable per diagnostics value.
+// (unlimited), the record variable t and ps3d's 40 statistics (36-75, 453-798)
+// in ps3d's order, then the extensions helicity, dt and prefactor.
 //
-// Format (all integers big-endian): magic "CDF\2", numrecs, dimension list,
-// global attribute list, variable list (name, dimids, attributes, type, vsize,
-// 64-bit begin), non-record data, then records holding every record variable in
-// definition order.
+// Format (all integers big-endian): magic "CDF" + version (2 or 5), numrecs,
+// dimension list, global attribute list, variable list (name, dimids, attributes,
+// type, vsize, 64-bit begin), non-record data, then records holding every record
+// variable in definition order.  Counts, lengths, dimension ids and vsize are
+// 4-byte in CDF-2 and 8-byte in CDF-5 ("NON_NEG" of the format specification).
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -53,10 +57,11 @@ struct Var {
     bool record = false;
 };
 struct Header {
-    std::vector<std::pair<std::string, uint32_t>> dims;   // length 0 = unlimited
+    int ver = 2;                                          // 2: CDF-2, 5: CDF-5
+    std::vector<std::pair<std::string, uint64_t>> dims;   // length 0 = unlimited
     std::vector<Att> gatts;
     std::vector<Var> vars;
-    uint32_t numrecs = 0;
+    uint64_t numrecs = 0;
     uint64_t recsize = 0;
     uint64_t data_start = 0;
 };
@@ -64,6 +69,7 @@ struct Header {
 // ---- big-endian buffer ----
 struct Out {
     std::vector<unsigned char> b;
+    int ver = 2;
     void u32(uint32_t v) {
         for (int i = 3; i >= 0; --i) b.push_back((unsigned char)(v >> (8 * i)));
     }
@@ -75,11 +81,14 @@ struct Out {
         std::memcpy(&v, &d, 8);
         u64(v);
     }
+    void nn(uint64_t v) {   // NON_NEG
+        if (ver == 5) u64(v); else u32((uint32_t)v);
+    }
     void pad() {
         while (b.size() % 4) b.push_back(0);
     }
     void name(const std::string &s) {
-        u32((uint32_t)s.size());
+        nn(s.size());
         b.insert(b.end(), s.begin(), s.end());
         pad();
     }
@@ -89,6 +98,7 @@ struct In {
     const unsigned char *p;
     size_t n, o = 0;
     bool ok = true;
+    int ver = 2;
     uint32_t u32() {
         if (o + 4 > n) { ok = false; return 0; }
         uint32_t v = 0;
@@ -107,9 +117,10 @@ struct In {
         std::memcpy(&d, &v, 8);
         return d;
     }
+    uint64_t nn() { return ver == 5 ? u64() : u32(); }
     void skip_pad() { o = (o + 3) & ~(size_t)3; }
     std::string name() {
-        const uint32_t len = u32();
+        const uint64_t len = nn();
         if (!ok || o + len > n) { ok = false; return {}; }
         std::string s((const char *)p + o, len);
         o += len;
@@ -119,25 +130,25 @@ struct In {
 };
 
 void put_atts(Out &w, const std::vector<Att> &atts) {
-    if (atts.empty()) {
+    if (atts.empty()) {   // ABSENT
         w.u32(0);
-        w.u32(0);
+        w.nn(0);
         return;
     }
     w.u32(TAG_ATT);
-    w.u32((uint32_t)atts.size());
+    w.nn(atts.size());
     for (const Att &a : atts) {
         w.name(a.name);
         w.u32((uint32_t)a.type);
         if (a.type == NC_CHAR) {
-            w.u32((uint32_t)a.text.size());
+            w.nn(a.text.size());
             w.b.insert(w.b.end(), a.text.begin(), a.text.end());
             w.pad();
         } else if (a.type == NC_INT) {
-            w.u32((uint32_t)a.ints.size());
+            w.nn(a.ints.size());
             for (int32_t v : a.ints) w.u32((uint32_t)v);
         } else {
-            w.u32((uint32_t)a.dbls.size());
+            w.nn(a.dbls.size());
             for (double v : a.dbls) w.f64(v);
         }
     }
@@ -145,23 +156,24 @@ void put_atts(Out &w, const std::vector<Att> &atts) {
 
 std::vector<Att> get_atts(In &r) {
     std::vector<Att> out;
-    const uint32_t tag = r.u32(), n = r.u32();
+    const uint32_t tag = r.u32();
+    const uint64_t n = r.nn();
     if (tag == 0) return out;
     if (tag != TAG_ATT) { r.ok = false; return out; }
-    for (uint32_t i = 0; i < n && r.ok; ++i) {
+    for (uint64_t i = 0; i < n && r.ok; ++i) {
         Att a;
         a.name = r.name();
         a.type = (int)r.u32();
-        const uint32_t m = r.u32();
+        const uint64_t m = r.nn();
         if (a.type == NC_CHAR) {
             if (r.o + m > r.n) { r.ok = false; break; }
             a.text.assign((const char *)r.p + r.o, m);
             r.o += m;
             r.skip_pad();
         } else if (a.type == NC_INT) {
-            for (uint32_t k = 0; k < m; ++k) a.ints.push_back((int32_t)r.u32());
+            for (uint64_t k = 0; k < m && r.ok; ++k) a.ints.push_back((int32_t)r.u32());
         } else if (a.type == NC_DOUBLE) {
-            for (uint32_t k = 0; k < m; ++k) a.dbls.push_back(r.f64());
+            for (uint64_t k = 0; k < m && r.ok; ++k) a.dbls.push_back(r.f64());
         } else {
             r.ok = false;
         }
@@ -181,24 +193,26 @@ std::vector<unsigned char> encode(Header &h) {
     // two passes: the header size does not depend on the begin values
     for (int pass = 0; pass < 2; ++pass) {
         Out w;
-        w.b = {'C', 'D', 'F', 2};
-        w.u32(h.numrecs);
+        w.ver = h.ver;
+        w.b = {'C', 'D', 'F', (unsigned char)h.ver};
+        w.nn(h.numrecs);
         w.u32(TAG_DIM);
-        w.u32((uint32_t)h.dims.size());
+        w.nn(h.dims.size());
         for (auto &d : h.dims) {
             w.name(d.first);
-            w.u32(d.second);
+            w.nn(d.second);
         }
         put_atts(w, h.gatts);
         w.u32(TAG_VAR);
-        w.u32((uint32_t)h.vars.size());
+        w.nn(h.vars.size());
         for (Var &v : h.vars) {
             w.name(v.name);
-            w.u32((uint32_t)v.dims.size());
-            for (int d : v.dims) w.u32((uint32_t)d);
+            w.nn(v.dims.size());
+            for (int d : v.dims) w.nn((uint64_t)d);
             put_atts(w, v.atts);
             w.u32((uint32_t)v.type);
-            w.u32((uint32_t)(v.vsize > 0xffffffffull ? 0xffffffffu : v.vsize));
+            // CDF-2 keeps 2^32 - 1 for a size its 32-bit field cannot hold
+            w.nn(h.ver == 2 && v.vsize > 0xffffffffull ? 0xffffffffull : v.vsize);
             w.u64(v.begin);
         }
         if (pass == 1) return w.b;
@@ -223,27 +237,30 @@ std::vector<unsigned char> encode(Header &h) {
 
 int decode(const std::vector<unsigned char> &buf, Header &h) {
     In r{buf.data(), buf.size()};
-    if (buf.size() < 8 || std::memcmp(buf.data(), "CDF\2", 4) != 0) return nfail("not a NetCDF 64-bit offset file");
+    if (buf.size() < 8 || std::memcmp(buf.data(), "CDF", 3) != 0 || (buf[3] != 2 && buf[3] != 5))
+        return nfail("not a NetCDF 64-bit offset (CDF-2) or 64-bit data (CDF-5) file");
+    h.ver = r.ver = buf[3];
     r.o = 4;
-    h.numrecs = r.u32();
-    uint32_t tag = r.u32(), n = r.u32();
+    h.numrecs = r.nn();
+    uint32_t tag = r.u32();
+    uint64_t n = r.nn();
     if (tag == TAG_DIM)
-        for (uint32_t i = 0; i < n && r.ok; ++i) {
+        for (uint64_t i = 0; i < n && r.ok; ++i) {
             std::string nm = r.name();
-            h.dims.push_back({nm, r.u32()});
+            h.dims.push_back({nm, r.nn()});
         }
     h.gatts = get_atts(r);
     tag = r.u32();
-    n = r.u32();
+    n = r.nn();
     if (tag == TAG_VAR)
-        for (uint32_t i = 0; i < n && r.ok; ++i) {
+        for (uint64_t i = 0; i < n && r.ok; ++i) {
             Var v;
             v.name = r.name();
-            const uint32_t nd = r.u32();
-            for (uint32_t k = 0; k < nd; ++k) v.dims.push_back((int)r.u32());
+            const uint64_t nd = r.nn();
+            for (uint64_t k = 0; k < nd && r.ok; ++k) v.dims.push_back((int)r.nn());
             v.atts = get_atts(r);
             v.type = (int)r.u32();
-            v.vsize = r.u32();
+            v.vsize = r.nn();
             v.begin = r.u64();
             for (int d : v.dims)
                 if (d < 0 || d >= (int)h.dims.size()) r.ok = false;
@@ -306,6 +323,15 @@ double from_be(const unsigned char *p) {
     return d;
 }
 
+// numrecs + 1 in the file's header (4 bytes in CDF-2, 8 in CDF-5, after the magic)
+bool bump_numrecs(FILE *f, const Header &h) {
+    const uint64_t nr = h.numrecs + 1;
+    const int nb = h.ver == 5 ? 8 : 4;
+    unsigned char b[8];
+    for (int i = 0; i < nb; ++i) b[i] = (unsigned char)(nr >> (8 * (nb - 1 - i)));
+    return fseeko(f, 4, SEEK_SET) == 0 && std::fwrite(b, 1, nb, f) == (size_t)nb;
+}
+
 // append one record: values[k] = data of the k-th record variable (npts doubles each,
 // already in file order)
 int append_record(const char *path, const std::vector<std::vector<double>> &values) {
@@ -329,24 +355,9 @@ int append_record(const char *path, const std::vector<std::vector<double>> &valu
         be_doubles(values[k].data(), values[k].size(), buf);
         ok = fseeko(f, (off_t)off, SEEK_SET) == 0 && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
     }
-    if (ok) {
-        const uint32_t nr = h.numrecs + 1;
-        unsigned char b[4] = {(unsigned char)(nr >> 24), (unsigned char)(nr >> 16), (unsigned char)(nr >> 8),
-                              (unsigned char)nr};
-        ok = fseeko(f, 4, SEEK_SET) == 0 && std::fwrite(b, 1, 4, f) == 4;
-    }
+    if (ok) ok = bump_numrecs(f, h);
     std::fclose(f);
     return ok ? 0 : nfail("write failed");
-}
-
-// (x, y, z) host layout -> (z, y, x) file order
-void to_file_order(const double *src, int nx, int ny, int nz1, std::vector<double> &dst) {
-    dst.resize((size_t)nx * ny * nz1);
-    for (int ix = 0; ix < nx; ++ix)
-        for (int iy = 0; iy < ny; ++iy) {
-            const double *c = src + ((size_t)ix * ny + iy) * nz1;
-            for (int iz = 0; iz < nz1; ++iz) dst[((size_t)iz * ny + iy) * nx + ix] = c[iz];
-        }
 }
 
 const char *FIELD_NAMES[8] = {"x_velocity", "y_velocity", "z_velocity", "x_vorticity",
@@ -356,30 +367,56 @@ const char *FIELD_LONG[8] = {"x velocity component", "y velocity component", "z 
                              "pressure", "horizontal divergence"};
 const char *FIELD_UNIT[8] = {"m/s", "m/s", "m/s", "1/s", "1/s", "1/s", "m^2/s^2", "1/s"};
 
-// the diagnostics record (enum ps3d_diag) as field_stats variables: ps3d's name,
-// long name and unit where it has the quantity (field_diagnostics_netcdf.f90:453-798)
+// field_stats variables: t, ps3d's statistics in the order NC_KE .. NC_ROMAX with
+// its names, long names and units (field_diagnostics_netcdf.f90:36-75, 453-798),
+// then the extensions (a helicity series, the step and the prefactor)
 struct StatInfo {
     const char *name, *long_name, *unit;
 };
-const StatInfo STATS[PS3D_NDIAG] = {
+const StatInfo STATS[PS3D_NC_STATS_ROW] = {
     {"t", "time ", "seconds since 1970-01-01"},
-    {"dt", "time step", "s"},
     {"ke", "domain-averaged kinetic energy", "m^2/s^2"},
     {"en", "domain-averaged enstrophy", "1/s^2"},
-    {"helicity", "domain-averaged helicity", "m/s^2"},
     {"vortmax", "maximum vorticity magnitude", "1/s"},
     {"vortrms", "root-mean square vorticity", "1/s"},
     {"vorch", "characteristic vorticity", "1/s"},
     {"x_vormean", "mean x-vorticity", "1/s"},
     {"y_vormean", "mean y-vorticity", "1/s"},
     {"z_vormean", "mean z-vorticity", "1/s"},
+    {"kexy", "domain-averaged horizontal kinetic energy", "m^2/s^2"},
+    {"kez", "domain-averaged vertical kinetic energy", "m^2/s^2"},
+    {"enxy", "domain-averaged horizontal enstrophy", "1/s^2"},
+    {"enz", "domain-averaged vertical enstrophy", "1/s^2"},
+    {"x_vormin", "min x-vorticity", "1/s"},
+    {"y_vormin", "min y-vorticity", "1/s"},
+    {"z_vormin", "min z-vorticity", "1/s"},
+    {"x_vormax", "max x-vorticity", "1/s"},
+    {"y_vormax", "max y-vorticity", "1/s"},
+    {"z_vormax", "max z-vorticity", "1/s"},
+    {"enxy_max", "max horizontal enstrophy", "1/s"},
     {"gmax", "maximum gamma", "1/s"},
+    {"bfmax", "maximum buoyancy frequency", "1/s"},
     {"umax", "maximum x-velocity", "m/s"},
     {"vmax", "maximum y-velocity", "m/s"},
     {"wmax", "maximum z-velocity", "m/s"},
+    {"uoxmax", "upper surface maximum x-vorticity", "1/s"},
+    {"loxmax", "lower surface maximum x-vorticity", "1/s"},
+    {"uoymax", "upper surface maximum y-vorticity", "1/s"},
+    {"loymax", "lower surface maximum y-vorticity", "1/s"},
+    {"uozmax", "upper surface maximum z-vorticity", "1/s"},
+    {"lozmax", "lower surface maximum z-vorticity", "1/s"},
+    {"usuhmax", "upper surface maximum horizontal speed", "m/s"},
     {"usgmax", "upper surface maximum strain", "1/s"},
     {"lsgmax", "lower surface maximum strain", "1/s"},
+    {"uszrms", "rms of upper surface z-vorticity", "1/s"},
+    {"usdeltarms", "rms of upper surface horizontal divergence", "1/s"},
     {"rolling_mean_gmax", "rolling mean maximum gamma", "1/s"},
+    {"rolling_mean_bfmax", "rolling mean maximum buoyancy frequency", "1/s"},
+    {"ri_min", "minimum Richardson number", "1"},
+    {"ro_min", "minimum Rossby number", "1"},
+    {"ro_max", "maximum Rossby number", "1"},
+    {"helicity", "domain-averaged helicity", "m/s^2"},
+    {"dt", "time step", "s"},
     {"prefactor", "diffusion prefactor", "1"},
 };
 
@@ -390,12 +427,15 @@ extern "C" {
 const char *ps3d_nc_last_error(void) { return nc_err.c_str(); }
 
 int ps3d_nc_create_fields(const char *path, int nx, int ny, int nz, const double *lower, const double *extent,
-                          int with_derived) {
+                          int flags) {
+    const bool with_derived = (flags & PS3D_NC_DERIVED) != 0;
     if (!path || !lower || !extent || nx < 1 || ny < 1 || nz < 1) return nfail("invalid argument");
     const uint64_t vol = 8ull * nx * ny * (nz + 1);
-    if (vol > 0xffffffffull) return nfail("a field record exceeds 4 GiB: too large for the 64-bit offset format");
     Header h;
-    h.dims = {{"t", 0}, {"z", (uint32_t)(nz + 1)}, {"y", (uint32_t)ny}, {"x", (uint32_t)nx}};
+    // a variable's record must fit CDF-2's 32-bit size field (less 4 bytes of
+    // alignment slack), else the file is CDF-5
+    h.ver = (vol >= 0xfffffffcull || (flags & PS3D_NC_CDF5)) ? 5 : 2;
+    h.dims = {{"t", 0}, {"z", (uint64_t)(nz + 1)}, {"y", (uint64_t)ny}, {"x", (uint64_t)nx}};
     h.gatts = {catt("file_type", "fields"), catt("Conventions", "CF-1.9"),
                iatt("ncells", {nx, ny, nz}), datt("extent", {extent[0], extent[1], extent[2]}),
                datt("origin", {lower[0], lower[1], lower[2]})};
@@ -445,19 +485,39 @@ int ps3d_nc_append_fields(const char *path, double t, int nx, int ny, int nz, co
     Header h;
     int st = read_header(path, h);
     if (st) return st;
-    int nrec = 0;
-    for (const Var &v : h.vars) nrec += v.record;
-    const bool derived = nrec == 9;
+    std::vector<const Var *> rv;
+    for (const Var &v : h.vars)
+        if (v.record) rv.push_back(&v);
+    const bool derived = rv.size() == 9;
+    if (rv.size() != 7 && !derived) return nfail("record layout mismatch");
     if (derived && (!pres || !delta)) return nfail("this field file holds pressure and divergence: pass them");
-    const size_t hv = (size_t)nx * ny * (nz + 1);
-    std::vector<std::vector<double>> vals;
-    vals.push_back({t});
+    const size_t hv = (size_t)nx * ny * (nz + 1), plane = (size_t)nx * ny;
+    if (rv[1]->vsize != hv * 8) return nfail("grid differs from the file's");
     const double *src[8] = {vel, vel + hv, vel + 2 * hv, vor, vor + hv, vor + 2 * hv, pres, delta};
-    for (int k = 0; k < (derived ? 8 : 6); ++k) {
-        vals.emplace_back();
-        to_file_order(src[k], nx, ny, nz + 1, vals.back());
+    FILE *f = std::fopen(path, "r+b");
+    if (!f) return nfail(std::string("cannot open ") + path);
+    std::vector<unsigned char> buf;
+    bool ok;
+    {
+        be_doubles(&t, 1, buf);
+        ok = fseeko(f, (off_t)(rv[0]->begin + h.numrecs * h.recsize), SEEK_SET) == 0 &&
+             std::fwrite(buf.data(), 1, 8, f) == 8;
     }
-    return append_record(path, vals);
+    // each variable in file order (t, z, y, x), one z level at a time
+    std::vector<double> lev(plane);
+    for (int k = 0; k < (derived ? 8 : 6) && ok; ++k) {
+        const uint64_t base = rv[1 + k]->begin + h.numrecs * h.recsize;
+        ok = fseeko(f, (off_t)base, SEEK_SET) == 0;
+        for (int iz = 0; iz <= nz && ok; ++iz) {
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix) lev[(size_t)iy * nx + ix] = src[k][((size_t)ix * ny + iy) * (nz + 1) + iz];
+            be_doubles(lev.data(), plane, buf);
+            ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+        }
+    }
+    if (ok) ok = bump_numrecs(f, h);
+    std::fclose(f);
+    return ok ? 0 : nfail("write failed");
 }
 
 int ps3d_nc_read_fields(const char *path, int step, double *t, int nx, int ny, int nz, double *vel, double *vor) {
@@ -483,20 +543,24 @@ int ps3d_nc_read_fields(const char *path, int step, double *t, int nx, int ny, i
     auto read_var = [&](const char *name, double *dst, bool scalar) {
         for (const Var &v : h.vars) {
             if (v.name != name || !v.record) continue;
-            buf.resize(v.vsize);
             const uint64_t off = v.begin + (uint64_t)rec * h.recsize;
-            ok = ok && fseeko(f, (off_t)off, SEEK_SET) == 0 && std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+            ok = ok && fseeko(f, (off_t)off, SEEK_SET) == 0;
             if (!ok) return false;
             if (scalar) {
-                *dst = from_be(buf.data());
-            } else {
-                for (int iz = 0; iz <= nz; ++iz)
-                    for (int iy = 0; iy < ny; ++iy)
-                        for (int ix = 0; ix < nx; ++ix)
-                            dst[((size_t)ix * ny + iy) * (nz + 1) + iz] =
-                                from_be(&buf[(((size_t)iz * ny + iy) * nx + ix) * 8]);
+                buf.resize(8);
+                ok = std::fread(buf.data(), 1, 8, f) == 8;
+                if (ok) *dst = from_be(buf.data());
+                return ok;
             }
-            return true;
+            // one z level (ny, nx) at a time
+            buf.resize((size_t)nx * ny * 8);
+            for (int iz = 0; iz <= nz && ok; ++iz) {
+                ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+                for (int iy = 0; iy < ny && ok; ++iy)
+                    for (int ix = 0; ix < nx; ++ix)
+                        dst[((size_t)ix * ny + iy) * (nz + 1) + iz] = from_be(&buf[((size_t)iy * nx + ix) * 8]);
+            }
+            return ok;
         }
         return false;
     };
@@ -555,12 +619,14 @@ int ps3d_nc_grid(const char *path, int *ncells, double *extent, double *origin) 
     return 0;
 }
 
-int ps3d_nc_create_stats(const char *path) {
-    if (!path) return nfail("invalid argument");
+int ps3d_nc_create_stats(const char *path, int nx, int ny, int nz, const double *lower, const double *extent) {
+    if (!path || !lower || !extent || nx < 1 || ny < 1 || nz < 1) return nfail("invalid argument");
     Header h;
     h.dims = {{"t", 0}};
-    h.gatts = {catt("file_type", "field_stats"), catt("Conventions", "CF-1.9")};
-    for (int k = 0; k < PS3D_NDIAG; ++k) {
+    h.gatts = {catt("file_type", "field_stats"), catt("Conventions", "CF-1.9"),
+               iatt("ncells", {nx, ny, nz}), datt("extent", {extent[0], extent[1], extent[2]}),
+               datt("origin", {lower[0], lower[1], lower[2]})};
+    for (int k = 0; k < PS3D_NC_STATS_ROW; ++k) {
         h.vars.push_back(rec_var(STATS[k].name, {0}, STATS[k].long_name, STATS[k].unit));
         if (k == 0) {
             h.vars.back().atts.push_back(catt("standard_name", "time"));
@@ -570,11 +636,31 @@ int ps3d_nc_create_stats(const char *path) {
     return write_file(path, h);
 }
 
+int ps3d_nc_stats_row(const double *diag, const double *stats, double *row) {
+    if (!diag || !stats || !row) return nfail("invalid argument");
+    row[0] = diag[PS3D_D_T];
+    for (int k = 0; k < PS3D_NSTAT; ++k) row[1 + k] = stats[k];
+    // the values adapt hands to set_netcdf_field_diagnostic (advance.f90:188-193,
+    // 315-321, 366) come from the adapt record of the same time
+    const int ad[][2] = {{PS3D_ST_OMAX, PS3D_D_VORTMAX},     {PS3D_ST_ORMS, PS3D_D_VORTRMS},
+                         {PS3D_ST_OCHAR, PS3D_D_VORCH},      {PS3D_ST_OXMEAN, PS3D_D_VORMEAN_X},
+                         {PS3D_ST_OYMEAN, PS3D_D_VORMEAN_Y}, {PS3D_ST_OZMEAN, PS3D_D_VORMEAN_Z},
+                         {PS3D_ST_GMAX, PS3D_D_GGMAX},       {PS3D_ST_UMAX, PS3D_D_UMAX},
+                         {PS3D_ST_VMAX, PS3D_D_VMAX},        {PS3D_ST_WMAX, PS3D_D_WMAX},
+                         {PS3D_ST_USGMAX, PS3D_D_USGGMAX},   {PS3D_ST_LSGMAX, PS3D_D_LSGGMAX},
+                         {PS3D_ST_RGMAX, PS3D_D_RGMAX}};
+    for (const auto &m : ad) row[1 + m[0]] = diag[m[1]];
+    row[1 + PS3D_NSTAT] = diag[PS3D_D_HEL];
+    row[2 + PS3D_NSTAT] = diag[PS3D_D_DT];
+    row[3 + PS3D_NSTAT] = diag[PS3D_D_PREFACTOR];
+    return 0;
+}
+
 int ps3d_nc_append_stats(const char *path, const double *rows, int nrows) {
     if (!path || !rows || nrows < 0) return nfail("invalid argument");
     for (int r = 0; r < nrows; ++r) {
         std::vector<std::vector<double>> vals;
-        for (int k = 0; k < PS3D_NDIAG; ++k) vals.push_back({rows[(size_t)r * PS3D_NDIAG + k]});
+        for (int k = 0; k < PS3D_NC_STATS_ROW; ++k) vals.push_back({rows[(size_t)r * PS3D_NC_STATS_ROW + k]});
         int st = append_record(path, vals);
         if (st) return st;
     }

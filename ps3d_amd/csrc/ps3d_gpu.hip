@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,7 @@ struct ps3d_handle {
     FftPlan px{}, py{}, pz{};
     ZLaunch zl{};
     int TQ = 0, QPB = 0, nquads = 0;
+    bool zfuse = true;             // RK4 tendency pass fused with the next vor2vel pass
     LineCfg lx{}, ly{};
     // tiles of the register-holding line modes (x products pass lxm, x adapt pass lxr,
     // adapt-stage y pass lyr) and their block sizes: 1024 threads on the standard
@@ -90,6 +92,8 @@ struct ps3d_handle {
     float cnn_in[6]{}, cnn_out[3]{};
     bool have_grad = false;   // grad holds the physical y-derivatives of u, v, w (vor2vel, before source)
     bool have_derived = false;
+    bool derived_fresh = false;   // pres / delta belong to the state of the last vor2vel
+    double *stats_part = nullptr, *stats_out = nullptr;   // ps3d_gpu_field_stats, allocated on first use
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_steps = 0;
@@ -221,6 +225,8 @@ static void free_handle(ps3d_handle *h) {
     hipFree(h->pool);
     hipFree(h->pres);
     hipFree(h->delta);
+    hipFree(h->stats_part);
+    hipFree(h->stats_out);
     hipFree(h->scal);
     hipFree(h->party);
     hipFree(h->partx);
@@ -346,6 +352,8 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
         int var = 2;
         if (ev && ev[0] >= '1' && ev[0] <= '3') var = ev[0] - '0';
         h->zl.wave = (ps3d::zwave_supported(nz) && !off) ? var : 0;
+        const char *fz = std::getenv("PS3D_ZFUSE");
+        h->zfuse = !(fz && fz[0] == '0');
     }
 
     h->ly = line_cfg(ny, nz, g.zs, nx, (long)g.zs, (long)ny * g.zs);
@@ -672,6 +680,12 @@ static int products(ps3d_handle *h, int phys_in, double *const *dst, bool nn_exp
     return st;
 }
 
+// Whether RK4 stage `stage`'s tendency pass also runs the vor2vel z pass of the
+// next stage (wave kernels; PS3D_ZFUSE=0 keeps them apart, for comparison).
+static bool fuse_next(const ps3d_handle *h, int mode, int stage) {
+    return mode == ZT_RK4 && stage <= 3 && h->zl.wave != 0 && h->zfuse;
+}
+
 // One fused RK stage (impl_rk4) or cn2 iteration.  stage 1 includes adapt.
 static int fused_stage(ps3d_handle *h, int mode, int stage) {
     int st;
@@ -684,10 +698,12 @@ static int fused_stage(ps3d_handle *h, int mode, int stage) {
         if ((st = x_adapt(h, 0, true))) return st;
         rqp = h->svel;
     } else {
-        ZInvArgs z = zinv_args(h);
-        prof_begin(h, PS3D_K_ZINV);
-        HIPCHK(launch_zinv(h->zl, z, h->stream));
-        prof_end(h);
+        if (!fuse_next(h, mode, stage - 1)) {
+            ZInvArgs z = zinv_args(h);
+            prof_begin(h, PS3D_K_ZINV);
+            HIPCHK(launch_zinv(h->zl, z, h->stream));
+            prof_end(h);
+        }
         double *in[6] = {h->svel[0], h->svel[1], h->svel[2], h->vor[0], h->vor[1], h->vor[2]};
         double *out[6] = {h->vel[0], h->vel[1], h->vel[2], h->vor[0], h->vor[1], h->vor[2]};
         prof_begin(h, PS3D_K_YINV);
@@ -711,6 +727,12 @@ static int fused_stage(ps3d_handle *h, int mode, int stage) {
         }
         for (int c = 0; c < 3; ++c) t.extra[c] = h->mid[c];
     }
+    if (fuse_next(h, mode, stage)) {
+        prof_begin(h, PS3D_K_ZTEND_ZINV);
+        HIPCHK(launch_ztend_zinv_w(h->zl.nz, h->zl.wave, h->zl.nquads, t, zinv_args(h), h->stream));
+        prof_end(h);
+        return 0;
+    }
     prof_begin(h, PS3D_K_ZTEND);
     HIPCHK(launch_ztend(h->zl, t, h->stream));
     prof_end(h);
@@ -732,16 +754,56 @@ static int one_step(ps3d_handle *h) {
 }
 
 // host <-> device copies between ps3d's (0:nz, y, x) layout and the padded columns
+// Host <-> device copies between ps3d's (0:nz, y, x) layout and the padded columns.
+// A strided 2-D copy from pageable memory moves one short row at a time; instead
+// whole padded columns go through one pinned staging buffer of the process in large
+// contiguous copies, and the (un)padding is a memcpy per column on the host.
+namespace {
+constexpr size_t kStageBytes = 16u << 20;
+std::mutex g_stage_mu;
+double *g_stage = nullptr;
+}  // namespace
+
+static int with_stage(double **stage) {
+    if (!g_stage && hipHostMalloc((void **)&g_stage, kStageBytes, hipHostMallocDefault) != hipSuccess) {
+        g_stage = nullptr;
+        return fail(PS3D_E_NOMEM, "pinned staging buffer");
+    }
+    *stage = g_stage;
+    return 0;
+}
+
 static int copy_in(ps3d_handle *h, double *dst, const double *src) {
-    const size_t w = sizeof(double) * (h->g.nz + 1);
-    HIPCHK(hipMemcpy2DAsync(dst, sizeof(double) * h->g.zs, src, w, w, (size_t)h->g.nx * h->g.ny, hipMemcpyHostToDevice,
-                            h->stream));
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    double *stage;
+    int st;
+    if ((st = with_stage(&stage))) return st;
+    const size_t nz1 = (size_t)h->g.nz + 1, zs = h->g.zs, ncol = (size_t)h->g.nx * h->g.ny;
+    const size_t cpc = std::max<size_t>(1, kStageBytes / (sizeof(double) * zs));
+    for (size_t c0 = 0; c0 < ncol; c0 += cpc) {
+        const size_t nc = std::min(cpc, ncol - c0);
+        for (size_t c = 0; c < nc; ++c) {
+            std::memcpy(stage + c * zs, src + (c0 + c) * nz1, sizeof(double) * nz1);
+            std::memset(stage + c * zs + nz1, 0, sizeof(double) * (zs - nz1));
+        }
+        HIPCHK(hipMemcpyAsync(dst + c0 * zs, stage, sizeof(double) * nc * zs, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));   // the buffer is refilled next
+    }
     return 0;
 }
 static int copy_out(ps3d_handle *h, double *dst, const double *src) {
-    const size_t w = sizeof(double) * (h->g.nz + 1);
-    HIPCHK(hipMemcpy2DAsync(dst, w, src, sizeof(double) * h->g.zs, w, (size_t)h->g.nx * h->g.ny, hipMemcpyDeviceToHost,
-                            h->stream));
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    double *stage;
+    int st;
+    if ((st = with_stage(&stage))) return st;
+    const size_t nz1 = (size_t)h->g.nz + 1, zs = h->g.zs, ncol = (size_t)h->g.nx * h->g.ny;
+    const size_t cpc = std::max<size_t>(1, kStageBytes / (sizeof(double) * zs));
+    for (size_t c0 = 0; c0 < ncol; c0 += cpc) {
+        const size_t nc = std::min(cpc, ncol - c0);
+        HIPCHK(hipMemcpyAsync(stage, src + c0 * zs, sizeof(double) * nc * zs, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (size_t c = 0; c < nc; ++c) std::memcpy(dst + (c0 + c) * nz1, stage + c * zs, sizeof(double) * nz1);
+    }
     return 0;
 }
 
@@ -922,6 +984,7 @@ int ps3d_gpu_vor2vel(ps3d_handle *h) {
     h->have_phys = true;
     h->have_grad = true;
     h->have_adapt = false;
+    h->derived_fresh = false;
     return 0;
 }
 
@@ -1150,6 +1213,14 @@ int ps3d_gpu_profile_kernels(ps3d_handle *h, int nsteps, double *ms, int *launch
         bytes[PS3D_K_YFWD] = 6.0 * vol;         // 3 R + 3 W
         // z forward + update, per stage: rk4 15, 15, 15, 9 ; cn2 12, 9, 9
         bytes[PS3D_K_ZTEND] = (cn2 ? 10.0 : 13.5) * vol;
+        // fused: stage 1..3 tendency + update without the stage result (12) and the
+        // next stage's z inverse without its svor read (6 W); stage 4 alone is 9; the
+        // z inverse left is the adapt stage's (14)
+        bytes[PS3D_K_ZTEND_ZINV] = 18.0 * vol;
+        if (!cn2 && h->zl.wave && h->zfuse) {
+            bytes[PS3D_K_ZTEND] = 9.0 * vol;
+            bytes[PS3D_K_ZINV] = 14.0 * vol;
+        }
         bytes[PS3D_K_YINV_ADAPT] = 21.0 * vol;  // 9 R + 12 W
         // x adapt in a step: 11 R (u, w for their x derivatives, the y derivatives,
         // omega, the midpoints); the velocity maxima come from the products pass
@@ -1212,6 +1283,52 @@ int ps3d_gpu_derived(ps3d_handle *h) {
     HIPCHK(launch_zop(h->zl, zop_args(h, ZOP_POISSON, h->pres, h->pres), h->stream));
     if ((st = line_fields(h, 0, LM_INV, 1, p1, p1)) || (st = line_fields(h, 1, LM_INV, 1, p1, p1))) return st;
     h->have_derived = true;
+    h->derived_fresh = true;
+    return 0;
+}
+
+// field_stats record (enum ps3d_stat) of the physical state of the last vor2vel, on
+// the handle's stream: partials in part, the record in out (device memory).
+static int field_stats_launch(ps3d_handle *h, const double *diag, double *part, double *out) {
+    StatsArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.nx = h->g.nx;
+    a.ny = h->g.ny;
+    a.nz = h->g.nz;
+    a.zs = h->g.zs;
+    for (int c = 0; c < 3; ++c) {
+        a.vel[c] = h->vel[c];
+        a.vor[c] = h->vor[c];
+    }
+    a.delta = h->delta;
+    a.part = part;
+    a.nb = stats_blocks(h->g.nx, h->g.ny);
+    if (a.nb > kStatsMaxBlocks) return fail(PS3D_E_UNSUPPORTED, "field_stats: partition beyond the partials array");
+    StatsFin f;
+    std::memset(&f, 0, sizeof f);
+    f.part = part;
+    f.nb = a.nb;
+    f.ncelli = 1.0 / ((double)h->g.nx * h->g.ny * h->g.nz);
+    f.nxnyi = 1.0 / ((double)h->g.nx * h->g.ny);
+    f.fcor3 = h->p.f_cor[2];
+    static_assert(PS3D_NDIAG <= (int)(sizeof f.ad / sizeof f.ad[0]), "adapt record fits");
+    if (diag) std::memcpy(f.ad, diag, sizeof(double) * PS3D_NDIAG);
+    f.out = out;
+    HIPCHK(launch_stats(a, f, h->stream));
+    return 0;
+}
+
+int ps3d_gpu_field_stats(ps3d_handle *h, const double *diag, double *stats) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
+    if (!h || !stats) return fail(PS3D_E_ARG, "null argument");
+    if (!h->have_phys) return fail(PS3D_E_STATE, "field_stats needs a preceding vor2vel");
+    int st;
+    if (!h->derived_fresh && (st = ps3d_gpu_derived(h))) return st;
+    if (!h->stats_part) HIPCHK(hipMalloc((void **)&h->stats_part, sizeof(double) * kStatsMaxBlocks * ST_NP));
+    if (!h->stats_out) HIPCHK(hipMalloc((void **)&h->stats_out, sizeof(double) * PS3D_NSTAT));
+    if ((st = field_stats_launch(h, diag, h->stats_part, h->stats_out))) return st;
+    HIPCHK(hipMemcpyAsync(stats, h->stats_out, sizeof(double) * PS3D_NSTAT, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 

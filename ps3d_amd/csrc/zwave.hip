@@ -438,9 +438,11 @@ template <int NZ>
 struct WOps {
     WF<NZ> a, b;
 };
-template <int NZ>
+// KEEP: the next stage's input stays in registers (fused with the next vor2vel
+// pass), so every member -- also one that repeats another member's column -- forms it
+template <int NZ, bool KEEP = false>
 __device__ __attribute__((always_inline)) inline void wrk_load(const WC &c, const RkArgs &rk, int mode, int stage, int cc, WOps<NZ> &o) {
-    if (!c.store) return;
+    if (!KEEP && !c.store) return;
     // fixed destinations (a: q, qdi or qdf; b: qdf at RK4 stages 2 and 3)
     const bool two = mode != ZT_CN2 && (stage == 2 || stage == 3);
     const double *pa = (stage == 1) ? rk.q[cc] : (two ? rk.qdi[cc] : rk.qdf[cc]);
@@ -478,10 +480,12 @@ __device__ inline WRk wrk_setup(const WC &c, const RkArgs &rk, int mode, const Z
     return r;
 }
 
-template <int NZ>
+// KEEP (RK4 stages 1-3): the stage's result q -- the next stage's svor -- goes to
+// keep (every lane holds the top value) instead of memory; qdi / qdf are stored.
+template <int NZ, bool KEEP = false>
 __device__ __attribute__((always_inline)) inline void wrk_apply(const WC &c, const RkArgs &rk, int mode, int stage, int cc, const WF<NZ> &S,
-                                 const WOps<NZ> &o, const WRk &r, const ZTabs &tb) {
-    if (!c.store) return;
+                                 const WOps<NZ> &o, const WRk &r, const ZTabs &tb, WF<NZ> *keep = nullptr) {
+    if (!KEEP && !c.store) return;
     constexpr int NS = WDim<NZ>::S;
     const int l = c.lane;
     const double dt = r.dt;
@@ -509,26 +513,40 @@ __device__ __attribute__((always_inline)) inline void wrk_apply(const WC &c, con
     }
     const double dt2 = 0.5 * dt, dt3 = dt / 3.0, dt6 = dt / 6.0;
     const double em = r.em, epf = r.epf;
+    // with KEEP every lane forms the (wave-uniform) top value; lane 0 stores it
+    const bool st = c.store;
 #pragma unroll
     for (int s = 0; s <= NS; ++s) {
-        if (s == NS && l != 0) continue;
+        if (!KEEP && s == NS && l != 0) continue;
+        const bool mine = s < NS || l == 0;
         const int z = s < NS ? l + 64 * s : NZ;
         const double t = s < NS ? S.v[s % NS] : S.top;
         const double a0 = s < NS ? o.a.v[s % NS] : o.a.top;
         const size_t i = c.col + z;
+        double q;
         if (stage == 2 || stage == 3) {
             const double b0 = s < NS ? o.b.v[s % NS] : o.b.top;
             const double sq = epf * t;
-            rk.q[cc][i] = (stage == 2 ? em : em * em) * (a0 + (stage == 2 ? dt2 : dt) * sq);
-            rk.qdf[cc][i] = b0 + dt3 * sq;
+            q = (stage == 2 ? em : em * em) * (a0 + (stage == 2 ? dt2 : dt) * sq);
+            if (st && mine) rk.qdf[cc][i] = b0 + dt3 * sq;
         } else if (stage == 1) {
             const double sq = r.f0 * t;
-            rk.qdi[cc][i] = a0;
-            rk.q[cc][i] = (a0 + dt2 * sq) * em;
-            rk.qdf[cc][i] = a0 + dt6 * sq;
+            q = (a0 + dt2 * sq) * em;
+            if (st && mine) {
+                rk.qdi[cc][i] = a0;
+                rk.qdf[cc][i] = a0 + dt6 * sq;
+            }
         } else {
             const double sq = (epf * epf) * t;
-            rk.q[cc][i] = (em * em) * (a0 + dt6 * sq);
+            q = (em * em) * (a0 + dt6 * sq);
+        }
+        if constexpr (KEEP) {
+            if (s < NS)
+                keep->v[s % NS] = q;
+            else
+                keep->top = q;
+        } else {
+            rk.q[cc][i] = q;
         }
     }
 }
@@ -549,13 +567,13 @@ __device__ inline void wzero(WF<NZ> &F) {
 // OCC > 1 asks the compiler for OCC waves per SIMD (fewer registers, possibly
 // spills); OCC = 1 leaves it free.  Both variants are built; the launch shape
 // (and the ensemble tuner) choose.
-template <int NZ, int OCC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_zinv_w(ZInvArgs a) {
+// The body: svor from memory, or (reg != nullptr) the three components already in
+// this wave's registers (the fused tendency + next-stage pass below).
+template <int NZ>
+__device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &a, const WC &c, double2 *wsm,
+                                                                const WF<NZ> *reg) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
-    extern __shared__ double2 wsm[];
     const Grid g = a.g;
-    if (gated_off(g.gate)) return;
-    const WC c = make_wc(g, a.tb, (long)blockIdx.y * a.bt.vol);
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
     const WTw<P> tw = make_tw<P>(a.twz, c.lane);
     const double sc = a.zscale;
@@ -563,9 +581,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     const int m1 = c.m ^ 1, m2 = c.m ^ 2;
 
     WF<NZ> Xi, Et, C, D, E;
-    wload(c, a.svor[0], Xi);
-    wload(c, a.svor[1], Et);
-    wload(c, a.svor[2], C);
+    if (reg) {
+        Xi = reg[0];
+        Et = reg[1];
+        C = reg[2];
+    } else {
+        wload(c, a.svor[0], Xi);
+        wload(c, a.svor[1], Et);
+        wload(c, a.svor[2], C);
+    }
 
     // D = B_x - A_y (inversion.f90:41-45)
     xput(c, wsm, 0, Et);
@@ -751,17 +775,25 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     }
 }
 
+template <int NZ, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_zinv_w(ZInvArgs a) {
+    extern __shared__ double2 wsm[];
+    if (gated_off(a.g.gate)) return;
+    const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
+    zinv_body<NZ>(a, c, wsm, nullptr);
+}
+
 // ---------------------------------------------------------------------------
 // vorticity_tendency, vertical part (inversion.f90:298-371), fused with one RK4
 // substep or cn2 update; one wave per member.  Same contract as k_ztend.
 // ---------------------------------------------------------------------------
-template <int NZ, int OCC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_ztend_w(ZTendArgs a) {
+// The body; KEEP (RK4 stages 1-3): the stage's result stays in keep[3] for the
+// next stage's vor2vel pass instead of going to memory.
+template <int NZ, bool KEEP>
+__device__ __attribute__((always_inline)) inline void ztend_body(const ZTendArgs &a, const WC &c, double2 *wsm,
+                                                                 WF<NZ> *keep) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
-    extern __shared__ double2 wsm[];
     const Grid g = a.g;
-    if (gated_off(g.gate)) return;
-    const WC c = make_wc(g, a.tb, (long)blockIdx.y * a.bt.vol);
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
     const WTw<P> tw = make_tw<P>(a.twz, c.lane);
     const double sc = a.zscale;
@@ -791,9 +823,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
         wxform<NZ, WK_DST, WK_NONE>(c, Pp, none, ws, tw, sc);
     }
     // operands of the first component's update: in flight across the exchange
-    const bool upd = a.mode != ZT_STORE;
+    const bool upd = KEEP || a.mode != ZT_STORE;
     WOps<NZ> ops;
-    if (upd) wrk_load<NZ>(c, a.rk, a.mode, a.stage, 0, ops);
+    if (upd) wrk_load<NZ, KEEP>(c, a.rk, a.mode, a.stage, 0, ops);
     // dxi/dt = r_y - q_z ; deta/dt = p_z - r_x ; dzeta/dt = q_x - p_y (341-367)
     xput(c, wsm, 0, R);
     xput(c, wsm, 1, Q);
@@ -835,10 +867,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) {
         WOps<NZ> nxt;
-        if (cc < 2) wrk_load<NZ>(c, a.rk, a.mode, a.stage, cc + 1, nxt);
-        wrk_apply<NZ>(c, a.rk, a.mode, a.stage, cc, St[cc], ops, rr, a.tb);
+        if (cc < 2) wrk_load<NZ, KEEP>(c, a.rk, a.mode, a.stage, cc + 1, nxt);
+        wrk_apply<NZ, KEEP>(c, a.rk, a.mode, a.stage, cc, St[cc], ops, rr, a.tb, KEEP ? keep + cc : nullptr);
         if (cc < 2) ops = nxt;
     }
+}
+
+template <int NZ, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_ztend_w(ZTendArgs a) {
+    extern __shared__ double2 wsm[];
+    if (gated_off(a.g.gate)) return;
+    const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
+    ztend_body<NZ, false>(a, c, wsm, nullptr);
+}
+
+// RK4 stage s (1..3) of the tendency pass and the vor2vel z pass of stage s+1 in
+// one kernel: the new svor of each column never leaves the registers of the wave
+// that formed it (saves writing and re-reading three volumes per stage boundary).
+// One barrier separates the two partner exchanges' uses of the block's LDS.
+template <int NZ, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+k_ztend_zinv_w(ZTendArgs a, ZInvArgs b) {
+    extern __shared__ double2 wsm[];
+    if (gated_off(a.g.gate)) return;
+    const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
+    WF<NZ> q[3];
+    ztend_body<NZ, true>(a, c, wsm, q);
+    __syncthreads();
+    zinv_body<NZ>(b, c, wsm, q);
 }
 
 // Single z operators on one volume (kernel checks of the wave path).
@@ -928,6 +984,12 @@ hipError_t configure_zwave() {
         e = hipFuncSetAttribute((const void *)(k_ztend_w<NZ_, 3>), hipFuncAttributeMaxDynamicSharedMemorySize, \
                                 (int)zwave_lds_bytes(NZ_));                                                  \
     if (e == hipSuccess)                                                                                     \
+        e = hipFuncSetAttribute((const void *)(k_ztend_zinv_w<NZ_, 1>), hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)zwave_lds_bytes(NZ_));                                                  \
+    if (e == hipSuccess)                                                                                     \
+        e = hipFuncSetAttribute((const void *)(k_ztend_zinv_w<NZ_, 3>), hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)zwave_lds_bytes(NZ_));                                                  \
+    if (e == hipSuccess)                                                                                     \
         e = hipFuncSetAttribute((const void *)(k_zop_w<NZ_>), hipFuncAttributeMaxDynamicSharedMemorySize,    \
                                 (int)zwave_lds_bytes(NZ_));
     PS3D_ZW_SIZES(SETW)
@@ -962,6 +1024,21 @@ hipError_t launch_zinv_w(int nz, int variant, int nquads, const ZInvArgs &args, 
 }
 hipError_t launch_ztend_w(int nz, int variant, int nquads, const ZTendArgs &args, hipStream_t stream) {
     ZW_DISPATCH2(k_ztend_w, nquads, (args.bt.nb > 1 ? args.bt.nb : 1), args, variant == 1 || variant == 3);
+    return hipGetLastError();
+}
+hipError_t launch_ztend_zinv_w(int nz, int variant, int nquads, const ZTendArgs &t, const ZInvArgs &z,
+                               hipStream_t stream) {
+    const int nb = t.bt.nb > 1 ? t.bt.nb : 1;
+    const bool occ3 = variant == 3;
+    switch (nz * 10 + (occ3 ? 3 : 1)) {
+#define ZTZ(NZ_, O_)                                                                                         \
+    case NZ_ * 10 + O_:                                                                                      \
+        hipLaunchKernelGGL((k_ztend_zinv_w<NZ_, O_>), dim3(nquads, nb), dim3(256), zwave_lds_bytes(NZ_), stream, t, z); \
+        break;
+        ZTZ(128, 1) ZTZ(256, 1) ZTZ(512, 1) ZTZ(128, 3) ZTZ(256, 3) ZTZ(512, 3)
+#undef ZTZ
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 hipError_t launch_zop_w(int nz, int nquads, const ZOpArgs &args, hipStream_t stream) {

@@ -8,7 +8,8 @@
 ! read_netcdf_fields, src/utils/utils.f90:91-134, src/fields/field_netcdf.f90:276-433),
 ! runs the time loop of src/ps3d.f90:114-126 with the output rule of write_step
 ! (src/utils/utils.f90:53-89) and writes <basename>_fields.nc and
-! <basename>_field_stats.nc.  Files are NetCDF classic (64-bit offset).
+! <basename>_field_stats.nc (ps3d's 40 statistics per record).  Files are NetCDF
+! classic: 64-bit offset, or 64-bit data (CDF-5) where a field record exceeds 4 GiB.
 ! Extension: time%dt_fixed > 0 fixes the step (ps3d always adapts it).
 program ps3d_driver
     use iso_c_binding
@@ -56,7 +57,9 @@ program ps3d_driver
     type(c_ptr) :: h
     integer(c_int) :: ncells(3), nrec, st
     real(c_double) :: extent(3), origin(3), t, t0, tlast, dt, tstop
-    real(c_double) :: row(PS3D_NDIAG)
+    real(c_double) :: row(PS3D_NDIAG), stats(PS3D_NSTAT), srow(PS3D_NC_STATS_LEN)
+    logical :: wf, ws
+    integer(c_int) :: nmax
     ! steps queued per call of ps3d_gpu_advance_until (replays of one captured step)
     integer(c_int), parameter :: nchunk = 32
     real(c_double) :: rows(PS3D_NDIAG, nchunk)
@@ -181,39 +184,60 @@ program ps3d_driver
     call open_or_resume(sfile, .false.)
 
     ! --- run (src/ps3d.f90:114-126): advance = vor2vel, adapt, write_step, source, step ---
-    ! The loop test t < limit runs on the device: each call queues nchunk steps and
-    ! stops before the first one that would start at or after the next field output
-    ! time, where write_step writes the fields (t >= nfw * field_freq - eps, the
-    ! reference's t + eps >= nfw * field_freq).  Every step's record is the
-    ! field_stats row write_step would append at its time.
+    ! The loop test t < limit runs on the device: each call queues up to nchunk steps
+    ! and stops before the first one that would start at or after the next output
+    ! time of write_step (t + eps >= n * freq, src/utils/utils.f90:62-89).  At an
+    ! output time the physical fields (vor2vel, pressure and divergence) are formed
+    ! first; the fields are written, and the field_stats record takes the adapt
+    ! values of the step that starts there (its diagnostics row).
     do while (t < time%limit)
-        if (output%write_fields .and. t >= dble(nfw) * output%field_freq - eps) then
+        wf = output%write_fields .and. t >= dble(nfw) * output%field_freq - eps
+        ws = output%write_field_stats .and. t >= dble(nsfw) * output%field_stats_freq - eps
+        if (wf .or. ws) then
+            call gpu_check(ps3d_gpu_vor2vel(h), 'ps3d_gpu_vor2vel')
+            call gpu_check(ps3d_gpu_derived(h), 'ps3d_gpu_derived')
+        endif
+        if (wf) then
             call write_fields(t)
             nfw = nfw + 1
         endif
+        if (ws) then
+            call gpu_check(ps3d_gpu_field_stats(h, c_null_ptr, stats), 'ps3d_gpu_field_stats')
+            nsfw = nsfw + 1
+        endif
         tstop = time%limit
         if (output%write_fields) tstop = min(tstop, dble(nfw) * output%field_freq - eps)
-        call gpu_check(ps3d_gpu_advance_until(h, t, tstop, nchunk, rows, nd), 'ps3d_gpu_advance_until')
+        if (output%write_field_stats) tstop = min(tstop, dble(nsfw) * output%field_stats_freq - eps)
+        nmax = nchunk
+        if (tstop <= t) then
+            ! an output interval shorter than the step: one step, then write again
+            tstop = time%limit
+            nmax = 1
+        endif
+        call gpu_check(ps3d_gpu_advance_until(h, t, tstop, nmax, rows, nd), 'ps3d_gpu_advance_until')
         if (nd == 0 .and. t < tstop) then
             write(*, '(a)') 'Error: no step taken before the stop time.'
             error stop 1
         endif
+        if (ws .and. nd > 0) then
+            call nc_check(ps3d_nc_stats_row(rows(:, 1), stats, srow), 'write_netcdf_field_stats')
+            call nc_check(ps3d_nc_append_stats(cstr(sfile), srow, 1_c_int), 'write_netcdf_field_stats')
+        endif
         do k = 1, nd
-            if (output%write_field_stats .and. rows(1, k) + eps >= dble(nsfw) * output%field_stats_freq) then
-                call nc_check(ps3d_nc_append_stats(cstr(sfile), rows(:, k), 1_c_int), 'write_netcdf_field_stats')
-                nsfw = nsfw + 1
-            endif
             write(*, '(a, es22.14, a, es22.14)') ' At time', rows(1, k), ' and time step', rows(2, k)
         enddo
     enddo
 
     ! write_last_step (src/utils/utils.f90:53-60): forced write at the final time
     if (t > t0) then
+        call gpu_check(ps3d_gpu_vor2vel(h), 'ps3d_gpu_vor2vel')
+        call gpu_check(ps3d_gpu_derived(h), 'ps3d_gpu_derived')
         if (output%write_fields) call write_fields(t)
         if (output%write_field_stats) then
-            call gpu_check(ps3d_gpu_vor2vel(h), 'ps3d_gpu_vor2vel')
+            call gpu_check(ps3d_gpu_field_stats(h, c_null_ptr, stats), 'ps3d_gpu_field_stats')
             call gpu_check(ps3d_gpu_adapt(h, t, dt, row), 'ps3d_gpu_adapt')
-            call nc_check(ps3d_nc_append_stats(cstr(sfile), row, 1_c_int), 'write_netcdf_field_stats')
+            call nc_check(ps3d_nc_stats_row(row, stats, srow), 'write_netcdf_field_stats')
+            call nc_check(ps3d_nc_append_stats(cstr(sfile), srow, 1_c_int), 'write_netcdf_field_stats')
         endif
     endif
 
@@ -221,11 +245,10 @@ program ps3d_driver
 
 contains
 
-    ! write_netcdf_fields after vor2vel, with pressure and horizontal divergence
+    ! write_netcdf_fields of the state of the last vor2vel and derived (pressure and
+    ! horizontal divergence)
     subroutine write_fields(tt)
         real(c_double), intent(in) :: tt
-        call gpu_check(ps3d_gpu_vor2vel(h), 'ps3d_gpu_vor2vel')
-        call gpu_check(ps3d_gpu_derived(h), 'ps3d_gpu_derived')
         call gpu_check(ps3d_gpu_get(h, PS3D_VEL, vel), 'ps3d_gpu_get')
         call gpu_check(ps3d_gpu_get(h, PS3D_VOR, vor), 'ps3d_gpu_get')
         call gpu_check(ps3d_gpu_get(h, PS3D_PRES, pres), 'ps3d_gpu_get')
@@ -261,7 +284,8 @@ contains
             call nc_check(ps3d_nc_create_fields(cstr(path), p%nx, p%ny, p%nz, p%lower, p%extent, 1_c_int), &
                           'create_netcdf_field_file')
         else
-            call nc_check(ps3d_nc_create_stats(cstr(path)), 'create_netcdf_field_stats_file')
+            call nc_check(ps3d_nc_create_stats(cstr(path), p%nx, p%ny, p%nz, p%lower, p%extent), &
+                          'create_netcdf_field_stats_file')
         endif
     end subroutine
 

@@ -15,8 +15,12 @@ module ps3d_gpu
                                  PS3D_PRE_ROLL_MEAN_MAX_STRAIN = 3, PS3D_PRE_MAX_STRAIN = 4, &
                                  PS3D_PRE_US_MAX_STRAIN = 5
     integer(c_int), parameter :: PS3D_LS_KOLMOGOROV = 0, PS3D_LS_GEOPHYSICAL = 1
-    ! length of a diagnostics record (enum ps3d_diag)
+    ! length of a diagnostics record (enum ps3d_diag), of ps3d's field_stats record
+    ! (enum ps3d_stat) and of one field_stats file record (t, the statistics,
+    ! helicity, dt, prefactor: PS3D_NC_STATS_ROW of ps3d_nc.h)
     integer, parameter :: PS3D_NDIAG = 19
+    integer, parameter :: PS3D_NSTAT = 40
+    integer, parameter :: PS3D_NC_STATS_LEN = PS3D_NSTAT + 4
 
     type, bind(C) :: ps3d_params
         integer(c_int) :: nx, ny, nz
@@ -77,6 +81,12 @@ module ps3d_gpu
             real(c_double), intent(out) :: diag(*)
             integer(c_int), intent(out) :: nsteps
         end function
+        integer(c_int) function ps3d_gpu_field_stats(h, diag, stats) bind(C)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            type(c_ptr), value :: diag
+            real(c_double), intent(out) :: stats(*)
+        end function
         integer(c_int) function ps3d_gpu_get(h, which, f) bind(C)
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: h
@@ -107,10 +117,10 @@ module ps3d_gpu
             type(c_ptr), value :: vel
             real(c_double), intent(out) :: vor(*)
         end function
-        integer(c_int) function ps3d_nc_create_fields(path, nx, ny, nz, lower, extent, with_derived) bind(C)
+        integer(c_int) function ps3d_nc_create_fields(path, nx, ny, nz, lower, extent, flags) bind(C)
             import :: c_int, c_char, c_double
             character(kind=c_char), intent(in) :: path(*)
-            integer(c_int), value :: nx, ny, nz, with_derived
+            integer(c_int), value :: nx, ny, nz, flags
             real(c_double), intent(in) :: lower(3), extent(3)
         end function
         integer(c_int) function ps3d_nc_append_fields(path, t, nx, ny, nz, vel, vor, pres, delta) bind(C)
@@ -120,9 +130,16 @@ module ps3d_gpu
             integer(c_int), value :: nx, ny, nz
             real(c_double), intent(in) :: vel(*), vor(*), pres(*), delta(*)
         end function
-        integer(c_int) function ps3d_nc_create_stats(path) bind(C)
-            import :: c_int, c_char
+        integer(c_int) function ps3d_nc_create_stats(path, nx, ny, nz, lower, extent) bind(C)
+            import :: c_int, c_char, c_double
             character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), value :: nx, ny, nz
+            real(c_double), intent(in) :: lower(3), extent(3)
+        end function
+        integer(c_int) function ps3d_nc_stats_row(diag, stats, row) bind(C)
+            import :: c_int, c_double
+            real(c_double), intent(in) :: diag(*), stats(*)
+            real(c_double), intent(out) :: row(*)
         end function
         integer(c_int) function ps3d_nc_append_stats(path, rows, nrows) bind(C)
             import :: c_int, c_char, c_double

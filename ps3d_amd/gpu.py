@@ -27,6 +27,15 @@ DIAG_NAMES = ["t", "dt", "ke", "en", "hel", "vortmax", "vortrms", "vorch",
               "vormean_x", "vormean_y", "vormean_z", "ggmax", "umax", "vmax",
               "wmax", "usggmax", "lsggmax", "rgmax", "prefactor"]
 
+# ps3d's field_stats variables in the order of NC_KE .. NC_ROMAX
+# (src/fields/field_diagnostics_netcdf.f90:36-75), enum ps3d_stat
+STAT_NAMES = ["ke", "en", "vortmax", "vortrms", "vorch", "x_vormean", "y_vormean", "z_vormean",
+              "kexy", "kez", "enxy", "enz", "x_vormin", "y_vormin", "z_vormin",
+              "x_vormax", "y_vormax", "z_vormax", "enxy_max", "gmax", "bfmax", "umax", "vmax", "wmax",
+              "uoxmax", "loxmax", "uoymax", "loymax", "uozmax", "lozmax", "usuhmax", "usgmax", "lsgmax",
+              "uszrms", "usdeltarms", "rolling_mean_gmax", "rolling_mean_bfmax", "ri_min", "ro_min", "ro_max"]
+NSTAT = len(STAT_NAMES)
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 
@@ -67,6 +76,7 @@ def lib():
         L.ps3d_gpu_profile_kernels.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.ps3d_gpu_resample.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.ps3d_gpu_viscosity.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ps3d_gpu_field_stats.argtypes = [C.c_void_p, C.c_void_p, _dp]
         L.ps3d_gpu_create_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.ps3d_gpu_batch_size.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.ps3d_gpu_set_vorticity_batch.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -158,6 +168,15 @@ class Ps3dGpu:
 
     def source(self):
         _check(lib().ps3d_gpu_source(self.h))
+
+    def field_stats(self, diag=None):
+        """ps3d's field_stats record (STAT_NAMES order) of the state of the last
+        vor2vel; diag: the adapt record of the same time (adapt() or the step's
+        row of advance_n), whose entries ps3d's adapt hands to the stats."""
+        out = np.zeros(NSTAT)
+        d = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
+        _check(lib().ps3d_gpu_field_stats(self.h, None if d is None else d.ctypes.data, out))
+        return out
 
     def viscosity(self):
         """(visc, prefactor of the last adapt): column (k, l) decays at the rate

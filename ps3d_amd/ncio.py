@@ -1,5 +1,6 @@
-"""ps3d's field and field_stats NetCDF files (classic 64-bit offset format),
-written by the library's host code (include/ps3d_nc.h).
+"""ps3d's field and field_stats NetCDF files (classic data model: the 64-bit
+offset format, or the 64-bit data format CDF-5 where a field record exceeds
+4 GiB), written by the library's host code (include/ps3d_nc.h).
 
 FieldFile mirrors src/fields/field_netcdf.f90 (create_netcdf_field_file,
 write_netcdf_fields, read_netcdf_fields) and StatsFile mirrors
@@ -13,7 +14,11 @@ import os
 
 import numpy as np
 
-from .gpu import DELTA, NDIAG, PRES, VEL, VOR, lib
+from .gpu import DELTA, PRES, STAT_NAMES, VEL, VOR, lib
+
+# one field_stats record: t, ps3d's 40 statistics, then helicity, dt, prefactor
+STATS_ROW_NAMES = ["t"] + STAT_NAMES + ["helicity", "dt", "prefactor"]
+STATS_ROW = len(STATS_ROW_NAMES)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -36,7 +41,8 @@ def _L():
                                           C.c_int, C.c_void_p, _dp]
         L.ps3d_nc_records.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.ps3d_nc_grid.argtypes = [C.c_char_p, _ip, _dp, _dp]
-        L.ps3d_nc_create_stats.argtypes = [C.c_char_p]
+        L.ps3d_nc_create_stats.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.ps3d_nc_stats_row.argtypes = [_dp, _dp, _dp]
         L.ps3d_nc_append_stats.argtypes = [C.c_char_p, _dp, C.c_int]
         _ready = True
     return L
@@ -82,7 +88,9 @@ class FieldFile:
     """basename_fields.nc: velocity, vorticity and, with derived=True, pressure and
     horizontal divergence, one record per write."""
 
-    def __init__(self, basename, nx, ny, nz, lower, extent, derived=True):
+    def __init__(self, basename, nx, ny, nz, lower, extent, derived=True, cdf5=False):
+        """cdf5=True writes the 64-bit data format even where every record fits
+        the 64-bit offset one (which grids of 4 GiB records always use)."""
         self.path = basename + "_fields.nc"
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.derived = bool(derived)
@@ -92,7 +100,7 @@ class FieldFile:
             _check(_L().ps3d_nc_create_fields(
                 _b(self.path), self.nx, self.ny, self.nz,
                 np.ascontiguousarray(lower, dtype=np.float64),
-                np.ascontiguousarray(extent, dtype=np.float64), int(self.derived)))
+                np.ascontiguousarray(extent, dtype=np.float64), int(self.derived) | (2 if cdf5 else 0)))
 
     @classmethod
     def for_model(cls, basename, model, derived=True):
@@ -141,23 +149,46 @@ def read_fields(path, nx, ny, nz, step=-1, velocity=False):
     return (t.value, vor, vel) if velocity else (t.value, vor)
 
 
-class StatsFile:
-    """basename_field_stats.nc: one record per diagnostics row (NDIAG doubles, t first)."""
+def stats_row(diag, stats):
+    """A field_stats record from an adapt record (NDIAG) and the field statistics
+    of the same state (NSTAT, Ps3dGpu.field_stats): ps3d_nc_stats_row."""
+    row = np.zeros(STATS_ROW)
+    _check(_L().ps3d_nc_stats_row(np.ascontiguousarray(diag, dtype=np.float64),
+                                  np.ascontiguousarray(stats, dtype=np.float64), row))
+    return row
 
-    def __init__(self, basename):
+
+class StatsFile:
+    """basename_field_stats.nc: one record (STATS_ROW doubles, t first) per write."""
+
+    def __init__(self, basename, nx, ny, nz, lower, extent):
         self.path = basename + "_field_stats.nc"
         t = _resume(self.path)
         self.restart_time = -1.0 if t is None else t
         if t is None:
-            _check(_L().ps3d_nc_create_stats(_b(self.path)))
+            _check(_L().ps3d_nc_create_stats(_b(self.path), int(nx), int(ny), int(nz),
+                                             np.ascontiguousarray(lower, dtype=np.float64),
+                                             np.ascontiguousarray(extent, dtype=np.float64)))
+
+    @classmethod
+    def for_model(cls, basename, model):
+        p = model.p
+        return cls(basename, p.nx, p.ny, p.nz, list(p.lower), list(p.extent))
 
     def write(self, rows):
         """write_netcdf_field_stats (field_diagnostics_netcdf.f90:211-252) for each
-        row whose time is later than the file's last record; returns rows written."""
+        record whose time is later than the file's last one; returns records written."""
         rows = np.atleast_2d(np.asarray(rows, dtype=np.float64))
-        if rows.shape[1] != NDIAG:
-            raise ValueError("diagnostics rows must have %d entries" % NDIAG)
+        if rows.shape[1] != STATS_ROW:
+            raise ValueError("field_stats records must have %d entries" % STATS_ROW)
         rows = np.ascontiguousarray(rows[rows[:, 0] > self.restart_time])
         if len(rows):
             _check(_L().ps3d_nc_append_stats(_b(self.path), rows, len(rows)))
         return len(rows)
+
+    def write_model(self, model, diag):
+        """The record of a Ps3dGpu's state after vor2vel, with diag the adapt record
+        of the same time; returns False when skipped."""
+        if diag[0] <= self.restart_time:
+            return False
+        return self.write(stats_row(diag, model.field_stats(diag))) == 1

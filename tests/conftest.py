@@ -20,4 +20,8 @@ def oracle_mod():
         O.lib()
     except RuntimeError as e:
         pytest.skip(str(e))
+    # the oracle's OpenMP loops cost more than they save on the small grids of the
+    # CPU checks (and oversubscribe the host under parallel test workers); tests that
+    # step large grids set their own thread count
+    O.set_threads(1)
     return O

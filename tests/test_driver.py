@@ -80,8 +80,8 @@ def test_driver_run_matches_the_oracle(tmp_path, oracle_mod, dt_fixed, limit):
     base = str(tmp_path / "beltrami_32")
     cfg = tmp_path / "beltrami_32.config"
     cfg.write_text(CONFIG.format(field=ic.path, base=base).replace(
-        "time%limit              = 0.05", "time%limit              = %r" % limit).replace(
-        "time%dt_fixed           = 0.01", "time%dt_fixed           = %r" % dt_fixed))
+        "time%limit              = 0.05", "time%limit              = " + repr(limit)).replace(
+        "time%dt_fixed           = 0.01", "time%dt_fixed           = " + repr(dt_fixed)))
     print("driver run dt_fixed=%g limit=%g" % (dt_fixed, limit), flush=True)
     r = run_driver(str(cfg))
     assert r.returncode == 0, r.stdout + r.stderr
@@ -92,21 +92,42 @@ def test_driver_run_matches_the_oracle(tmp_path, oracle_mod, dt_fixed, limit):
     o = oracle_mod.Oracle(p)
     o.set_vorticity(vor0)
     eps = np.finfo(float).eps
-    ref, rows = [], []
+    ref, recs = [], []
     t, nfw, nsfw = 0.0, 0, 0
+    adapt_slots = {"vortmax": 5, "vortrms": 6, "vorch": 7, "x_vormean": 8, "y_vormean": 9, "z_vormean": 10,
+                   "gmax": 11, "umax": 12, "vmax": 13, "wmax": 14, "usgmax": 15, "lsgmax": 16,
+                   "rolling_mean_gmax": 17}
+
+    def record(stats, row):
+        # t, the 40 statistics with the adapt values of the step starting at t, then
+        # helicity, dt, prefactor (the file's record layout)
+        r = dict(zip(ncio.STATS_ROW_NAMES[1:41], stats))
+        for name, j in adapt_slots.items():
+            r[name] = row[j]
+        r.update(t=row[0], helicity=row[4], dt=row[1], prefactor=row[18])
+        return [r[n] for n in ncio.STATS_ROW_NAMES]
+
     while t < limit:
-        if t + eps >= nfw * 0.02:
+        wf = t + eps >= nfw * 0.02
+        ws = t + eps >= nsfw * 0.01
+        if wf or ws:
             o.vor2vel()
+            o.derived()
+        if wf:
             ref.append((t, o.get(oracle_mod.VOR), o.get(oracle_mod.VEL)))
             nfw += 1
-        row = o.advance_n(1)[0]
-        t = o.t
-        if row[0] + eps >= nsfw * 0.01:
-            rows.append(row)
+        if ws:
+            stats = o.field_stats()
             nsfw += 1
+        row = o.advance_n(1)[0]
+        if ws:
+            recs.append(record(stats, row))
+        t = o.t
     o.vor2vel()
+    o.derived()
     ref.append((t, o.get(oracle_mod.VOR), o.get(oracle_mod.VEL)))
-    rows.append(o.adapt(t)[1])
+    stats = o.field_stats()
+    recs.append(record(stats, o.adapt(t)[1]))
     o.close()
     assert abs(t - limit) < 1e-15
     with netcdf_file(base + "_fields.nc", "r", mmap=False) as f:
@@ -118,10 +139,19 @@ def test_driver_run_matches_the_oracle(tmp_path, oracle_mod, dt_fixed, limit):
                                            atol=1e-12 * np.abs(vor[c]).max())
                 np.testing.assert_allclose(f.variables[x + "_velocity"][k], vel[c].T, rtol=0,
                                            atol=1e-12 * np.abs(vel[c]).max())
-    rows = np.array(rows)
+    recs = np.array(recs)
     with netcdf_file(base + "_field_stats.nc", "r", mmap=False) as f:
         tt = np.array(f.variables["t"].data)
-        assert len(tt) == len(rows)
-        np.testing.assert_allclose(tt, rows[:, 0], rtol=0, atol=1e-15)
-        for name, col in (("ke", 2), ("en", 3)):
-            np.testing.assert_allclose(np.array(f.variables[name].data), rows[:, col], rtol=1e-12)
+        assert len(tt) == len(recs)
+        np.testing.assert_allclose(tt, recs[:, 0], rtol=0, atol=1e-15)
+        assert list(f.variables) == ncio.STATS_ROW_NAMES
+        rms = np.abs(recs[:, ncio.STATS_ROW_NAMES.index("vortrms")]).max()
+        for j, name in enumerate(ncio.STATS_ROW_NAMES):
+            got = np.array(f.variables[name].data)
+            want = recs[:, j]
+            if not np.all(np.isfinite(want)):
+                np.testing.assert_array_equal(got, want, err_msg=name)     # ro with f_cor(3) = 0
+                continue
+            scale = max(np.abs(want).max(), rms if name in ("uozmax", "lozmax", "uszrms", "x_vormean",
+                                                            "y_vormean", "z_vormean") else 0.0, 1e-300)
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * scale, err_msg=name)

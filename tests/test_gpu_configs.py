@@ -138,7 +138,7 @@ def beltrami_component(fac, c, scale_alpha=False):
     return out
 
 
-@pytest.mark.parametrize("n", [512, 1024])
+@pytest.mark.parametrize("n", [512, pytest.param(1024, marks=pytest.mark.timeout(900))])
 def test_large_grid_closed_forms(G, n):
     """Two Beltrami modes with alpha = 9 and k^2 + l^2 = 80 (u x omega = 0) on the
     whole n^3 grid in HBM: vor2vel gives the closed-form velocity (<= 1e-12), the

@@ -28,15 +28,17 @@ def test_fields_and_stats_files_from_a_run(tmp_path):
     d.set_vorticity(ics.beltrami_multimode(n, n, n, seed=3))
     base = str(tmp_path / "beltrami")
     ff = ncio.FieldFile.for_model(base, d, derived=True)
-    sf = ncio.StatsFile(base)
+    sf = ncio.StatsFile.for_model(base, d)
     kept = []
     for chunk in range(2):
         print("chunk", chunk, flush=True)
-        rows = d.advance_n(3)
-        sf.write(rows)
+        d.advance_n(3)
         d.vor2vel()
+        _, drow = d.adapt(d.t)
+        assert sf.write_model(d, drow)
+        srow = ncio.stats_row(drow, d.field_stats(drow))
         assert ff.write_model(d)
-        kept.append((d.t, d.get(G.VEL), d.get(G.VOR), d.get(G.PRES), d.get(G.DELTA), rows))
+        kept.append((d.t, d.get(G.VEL), d.get(G.VOR), d.get(G.PRES), d.get(G.DELTA), srow))
     with netcdf_file(base + "_fields.nc", "r", mmap=False) as f:
         np.testing.assert_array_equal(np.array(f.variables["t"].data), [k[0] for k in kept])
         for r, (t, vel, vor, pres, delta, _) in enumerate(kept):
@@ -46,9 +48,10 @@ def test_fields_and_stats_files_from_a_run(tmp_path):
             np.testing.assert_array_equal(f.variables["pressure"][r], pres.T)
             np.testing.assert_array_equal(f.variables["horizontal_divergence"][r], delta.T)
     with netcdf_file(base + "_field_stats.nc", "r", mmap=False) as f:
-        allrows = np.concatenate([k[5] for k in kept])
-        np.testing.assert_array_equal(np.array(f.variables["ke"].data), allrows[:, 2])
-        np.testing.assert_array_equal(np.array(f.variables["t"].data), allrows[:, 0])
+        allrows = np.array([k[5] for k in kept])
+        for j, name in enumerate(ncio.STATS_ROW_NAMES):
+            np.testing.assert_array_equal(np.array(f.variables[name].data), allrows[:, j])
+        np.testing.assert_array_equal(np.array(f.variables["t"].data), [k[0] for k in kept])
 
     # restart from the last record: the vorticity read back sets up the same state
     t, vor = ncio.read_fields(base + "_fields.nc", n, n, n, step=-1)

@@ -157,34 +157,190 @@ def test_reader_reads_scipy_written_file(tmp_path):
         np.testing.assert_array_equal(vel[c], vols[x + "_velocity"][0].transpose(2, 1, 0))
 
 
-def test_field_record_over_4gib_is_rejected(tmp_path):
+class Cdf:
+    """A reader of the NetCDF classic formats written from the published format
+    specification (CDF-2: 4-byte NON_NEG counts and sizes; CDF-5: 8-byte), for the
+    64-bit data files scipy cannot open: header fields and record data."""
+
+    def __init__(self, path):
+        self.b = open(path, "rb").read()
+        assert self.b[:3] == b"CDF"
+        self.ver = self.b[3]
+        self.o = 4
+        self.numrecs = self.nn()
+        self.dims = self._list(0x0A, lambda: (self.name(), self.nn()))
+        self.gatts = dict(self._list(0x0C, self.att))
+        self.vars = {}
+        for _ in range(self._tag(0x0B)):
+            name = self.name()
+            dimids = [self.nn() for _ in range(self.nn())]
+            atts = dict(self._list(0x0C, self.att))
+            typ, vsize, begin = self.u32(), self.nn(), self.u64()
+            self.vars[name] = dict(dimids=dimids, atts=atts, type=typ, vsize=vsize, begin=begin)
+        rec = [v for v in self.vars.values() if v["dimids"] and self.dims[v["dimids"][0]][1] == 0]
+        self.recsize = sum(v["vsize"] for v in rec)
+
+    def _int(self, n):
+        v = int.from_bytes(self.b[self.o:self.o + n], "big")
+        self.o += n
+        return v
+
+    def u32(self):
+        return self._int(4)
+
+    def u64(self):
+        return self._int(8)
+
+    def nn(self):
+        return self.u64() if self.ver == 5 else self.u32()
+
+    def name(self):
+        n = self.nn()
+        s = self.b[self.o:self.o + n].decode()
+        self.o += (n + 3) // 4 * 4
+        return s
+
+    def att(self):
+        name, typ, n = self.name(), self.u32(), None
+        n = self.nn()
+        if typ == 2:
+            v = self.b[self.o:self.o + n].decode()
+            self.o += (n + 3) // 4 * 4
+        elif typ == 4:
+            v = [int.from_bytes(self.b[self.o + 4 * i:self.o + 4 * i + 4], "big", signed=True) for i in range(n)]
+            self.o += 4 * n
+        else:
+            v = list(np.frombuffer(self.b[self.o:self.o + 8 * n], dtype=">f8"))
+            self.o += 8 * n
+        return name, v
+
+    def _tag(self, tag):
+        t = self.u32()
+        n = self.nn()
+        assert t in (0, tag)
+        return n if t == tag else 0
+
+    def _list(self, tag, item):
+        return [item() for _ in range(self._tag(tag))]
+
+    def record(self, name, rec, shape):
+        v = self.vars[name]
+        off = v["begin"] + rec * self.recsize
+        return np.frombuffer(self.b[off:off + v["vsize"]], dtype=">f8").reshape(shape)
+
+
+def test_cdf2_reader_agrees_with_scipy(tmp_path):
+    """The spec reader above, on a CDF-2 file, reads what scipy reads."""
     from ps3d_amd import ncio
-    with pytest.raises(ncio.NcError, match="4 GiB"):
-        ncio.FieldFile(str(tmp_path / "big"), 1024, 1024, 1024, LOWER, EXTENT)
-    assert not os.path.exists(str(tmp_path / "big_fields.nc"))
+    base = str(tmp_path / "two")
+    ff = ncio.FieldFile(base, NX, NY, NZ, LOWER, EXTENT, derived=False)
+    t, vel, vor, _, _ = (0.5,) + _fields(7)
+    ff.write(t, vel, vor)
+    c = Cdf(base + "_fields.nc")
+    assert c.ver == 2 and c.numrecs == 1
+    r = _read(base + "_fields.nc")
+    for name in ("x_velocity", "z_vorticity"):
+        np.testing.assert_array_equal(c.record(name, 0, (NZ + 1, NY, NX)), r["vars"][name][0][0])
+
+
+def test_cdf5_field_file(tmp_path):
+    """A field file in the 64-bit data format: header fields as the specification
+    lays them out, records in (t, z, y, x) order, and the library reads it back."""
+    from ps3d_amd import ncio
+    base = str(tmp_path / "five")
+    ff = ncio.FieldFile(base, NX, NY, NZ, LOWER, EXTENT, derived=True, cdf5=True)
+    recs = [(0.0,) + _fields(11), (0.5,) + _fields(12)]
+    for t, vel, vor, p, d in recs:
+        assert ff.write(t, vel, vor, p, d)
+    c = Cdf(base + "_fields.nc")
+    assert c.ver == 5 and c.numrecs == 2
+    assert c.dims == [("t", 0), ("z", NZ + 1), ("y", NY), ("x", NX)]
+    assert c.gatts["file_type"] == "fields" and c.gatts["ncells"] == [NX, NY, NZ]
+    np.testing.assert_array_equal(c.gatts["extent"], EXTENT)
+    assert c.vars["x_velocity"]["vsize"] == 8 * NX * NY * (NZ + 1)
+    assert c.vars["x_velocity"]["atts"]["long_name"] == "x velocity component"
+    for k, (t, vel, vor, p, d) in enumerate(recs):
+        assert c.record("t", k, ())[()] == t
+        for comp, x in enumerate("xyz"):
+            np.testing.assert_array_equal(c.record(x + "_velocity", k, (NZ + 1, NY, NX)), vel[comp].T)
+            np.testing.assert_array_equal(c.record(x + "_vorticity", k, (NZ + 1, NY, NX)), vor[comp].T)
+        np.testing.assert_array_equal(c.record("pressure", k, (NZ + 1, NY, NX)), p.T)
+        tt, vor_b, vel_b = ncio.read_fields(base + "_fields.nc", NX, NY, NZ, step=k + 1, velocity=True)
+        assert tt == t
+        np.testing.assert_array_equal(vor_b, vor)
+        np.testing.assert_array_equal(vel_b, vel)
+    assert ncio.records(base + "_fields.nc") == (2, 0.5)
+
+
+def test_1024_cubed_field_file_is_cdf5(tmp_path):
+    """ps3d's 1024^3 grid: each field record is 8.6 GB, beyond CDF-2's 32-bit size
+    field, so the file is CDF-5 with the full 64-bit vsize; the header alone is
+    written (no record)."""
+    from ps3d_amd import ncio
+    base = str(tmp_path / "big")
+    ncio.FieldFile(base, 1024, 1024, 1024, LOWER, EXTENT)
+    c = Cdf(base + "_fields.nc")
+    assert c.ver == 5 and c.numrecs == 0
+    assert c.vars["z_vorticity"]["vsize"] == 8 * 1024 * 1024 * 1025
+    assert c.vars["x"]["vsize"] == 8 * 1024 and c.vars["z"]["vsize"] == 8 * 1025
+    np.testing.assert_allclose(np.frombuffer(c.b[c.vars["z"]["begin"]:c.vars["z"]["begin"] + 8 * 1025], ">f8"),
+                               LOWER[2] + np.arange(1025) * (EXTENT[2] / 1024), rtol=0, atol=1e-15)
+    # 512^3 records fit CDF-2
+    ncio.FieldFile(str(tmp_path / "mid"), 512, 512, 512, LOWER, EXTENT)
+    assert Cdf(str(tmp_path / "mid") + "_fields.nc").ver == 2
 
 
 def test_stats_file(tmp_path):
     from ps3d_amd import ncio
     base = str(tmp_path / "run")
-    sf = ncio.StatsFile(base)
-    rows = np.random.default_rng(5).standard_normal((4, gpu.NDIAG))
+    sf = ncio.StatsFile(base, NX, NY, NZ, LOWER, EXTENT)
+    rows = np.random.default_rng(5).standard_normal((4, ncio.STATS_ROW))
     rows[:, 0] = [0.0, 0.1, 0.2, 0.3]
     assert sf.write(rows[:3]) == 3
     # resume: rows at or before the last time are skipped
-    sf2 = ncio.StatsFile(base)
+    sf2 = ncio.StatsFile(base, NX, NY, NZ, LOWER, EXTENT)
     assert sf2.restart_time == 0.2
     assert sf2.write(rows) == 1
     r = _read(base + "_field_stats.nc")
     assert r["dims"] == {"t": None}
-    names = ["t", "dt", "ke", "en", "helicity", "vortmax", "vortrms", "vorch", "x_vormean",
-             "y_vormean", "z_vormean", "gmax", "umax", "vmax", "wmax", "usgmax", "lsgmax",
-             "rolling_mean_gmax", "prefactor"]
+    # ps3d's 40 statistics in its order (field_diagnostics_netcdf.f90:36-75), then the extensions
+    names = ["t", "ke", "en", "vortmax", "vortrms", "vorch", "x_vormean", "y_vormean", "z_vormean",
+             "kexy", "kez", "enxy", "enz", "x_vormin", "y_vormin", "z_vormin", "x_vormax", "y_vormax",
+             "z_vormax", "enxy_max", "gmax", "bfmax", "umax", "vmax", "wmax", "uoxmax", "loxmax", "uoymax",
+             "loymax", "uozmax", "lozmax", "usuhmax", "usgmax", "lsgmax", "uszrms", "usdeltarms",
+             "rolling_mean_gmax", "rolling_mean_bfmax", "ri_min", "ro_min", "ro_max",
+             "helicity", "dt", "prefactor"]
     assert sorted(r["vars"]) == sorted(names)
+    with netcdf_file(base + "_field_stats.nc", "r", mmap=False) as f:
+        assert list(f.variables) == names
     for k, n in enumerate(names):
         np.testing.assert_array_equal(r["vars"][n][0], rows[:, k])
     assert r["vars"]["ke"][2]["long_name"] in (b"domain-averaged kinetic energy", "domain-averaged kinetic energy")
     assert r["vars"]["en"][2]["units"] in (b"1/s^2", "1/s^2")
+    assert r["vars"]["usdeltarms"][2]["long_name"] in (b"rms of upper surface horizontal divergence",
+                                                       "rms of upper surface horizontal divergence")
+    attrs = {k.decode() if isinstance(k, bytes) else k: v for k, v in r["attrs"].items()}
+    assert attrs["file_type"] in (b"field_stats", "field_stats")
+    assert list(attrs["ncells"]) == [NX, NY, NZ]
+
+
+def test_stats_row_takes_adapt_values_from_the_adapt_record():
+    """ps3d_nc_stats_row: t and the entries adapt sets (advance.f90:188-193,
+    315-321, 366) come from the adapt record, the rest from the field statistics."""
+    from ps3d_amd import ncio
+    diag = 1000.0 + np.arange(gpu.NDIAG)
+    stats = -1.0 - np.arange(gpu.NSTAT)
+    row = ncio.stats_row(diag, stats)
+    d = dict(zip(ncio.STATS_ROW_NAMES, row))
+    adapt = {"vortmax": "vortmax", "vortrms": "vortrms", "vorch": "vorch", "x_vormean": "vormean_x",
+             "y_vormean": "vormean_y", "z_vormean": "vormean_z", "gmax": "ggmax", "umax": "umax", "vmax": "vmax",
+             "wmax": "wmax", "usgmax": "usggmax", "lsgmax": "lsggmax", "rolling_mean_gmax": "rgmax",
+             "t": "t", "helicity": "hel", "dt": "dt", "prefactor": "prefactor"}
+    for name in ncio.STATS_ROW_NAMES:
+        if name in adapt:
+            assert d[name] == diag[gpu.DIAG_NAMES.index(adapt[name])], name
+        else:
+            assert d[name] == stats[gpu.STAT_NAMES.index(name)], name
 
 
 def test_grid_attributes_read_back(tmp_path):

@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 from tests import test_oracle_pins_step as P
+from tests import test_oracle_stats as ST
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "ps3d_oracle.c")
@@ -85,6 +86,19 @@ MUTANTS = [
      lambda O: P.check_mean_restore(O, "impl-diff-rk4")),
     ("vorticity mean without 1/nz", "savg[nc] = 0.5 * (s[0] + s[nz]) + o->fnzi * sum;",
      "savg[nc] = 0.5 * (s[0] + s[nz]) + sum;", lambda O: P.check_mean_restore(O, "cn2")),
+    ("stats lid weight", "a0 += f[0] * f[0];", "a0 += 2.0 * f[0] * f[0];",
+     lambda O: ST.check_horizontal_vorticity_stats(O)),
+    ("stats kez component", "st[9] = trapezoid_sq(o, o->vel, 2, 2);", "st[9] = trapezoid_sq(o, o->vel, 0, 0);",
+     lambda O: ST.check_beltrami_stats(O)),
+    ("stats enxy_max without root", "const double he = sqrt(xi[i] * xi[i] + eta[i] * eta[i]);",
+     "const double he = xi[i] * xi[i] + eta[i] * eta[i];", lambda O: ST.check_horizontal_vorticity_stats(O)),
+    ("stats lids swapped", "lid[2 * c] = fmax(lid[2 * c], w3[c][t]);", "lid[2 * c] = fmax(lid[2 * c], w3[c][b]);",
+     lambda O: ST.check_horizontal_vorticity_stats(O)),
+    ("stats usuhmax without root", "st[30] = sqrt(uh);", "st[30] = uh;", lambda O: ST.check_beltrami_stats(O)),
+    ("stats uszrms over all cells", "st[33] = sqrt(sz / (double)(o->nx * o->ny));", "st[33] = sqrt(sz * o->ncelli);",
+     lambda O: ST.check_vortex_stats(O)),
+    ("stats rossby of the ratio", "st[38] = mn[2] / o->p.f_cor[2];", "st[38] = fmin(mn[2], mx[2]) / fabs(o->p.f_cor[2]);",
+     lambda O: ST.check_vortex_stats(O)),
 ]
 
 
@@ -141,5 +155,8 @@ def test_unmutated_copy_passes(oracle_mod, tmp_path):
         P.check_decay(oracle_mod, "cn2", "vorch", prediss=100.0)
         P.check_cn2_held_source(oracle_mod, "Hou & Li")
         P.check_mean_restore(oracle_mod, "impl-diff-rk4")
+        ST.check_beltrami_stats(oracle_mod)
+        ST.check_horizontal_vorticity_stats(oracle_mod)
+        ST.check_vortex_stats(oracle_mod)
     finally:
         oracle_mod.use_library_object(prev)
