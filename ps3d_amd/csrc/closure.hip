@@ -15,11 +15,13 @@
 // Operand lanes: A row = lane%16 (point), k = (K/4)(lane/16)+v (channel);
 // B column = lane%16 (output channel), same k; D row = 4(lane/16)+v, column = lane%16.
 // The input gradient is the same kernel with the weights transposed and the taps
-// mirrored; the weight gradient is an fp32 reduction over point chunks.
+// mirrored; the weight gradient is a matrix-core reduction over point chunks
+// (bf16 gradient x bf16 input, fp32 accumulation; an fp32 path is kept).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -236,6 +238,88 @@ __global__ void __launch_bounds__(256) k_wgrad(WgArgs a) {
     if (t == 13 && tid < COUT) a.partb[(size_t)blockIdx.x * COUT + tid] = bacc;
 }
 
+// The same partials on the matrix cores: per block one chunk of WG_CHUNK points
+// and one tap, bf16 operands (the layer's bf16 output gradient and input), fp32
+// accumulation.  dW[co][ci] = sum_p g[p][co] in[p + d_t][ci] is a 16 x 16 tile
+// product per wave with K = points: A (16 co x 32 p) and B (32 p x 16 ci) are read
+// from LDS copies transposed to [channel][point], so that each lane's 8 consecutive
+// points of one channel are one 16-byte read.  The bias gradient (tap 13) stays an
+// fp32 sum of the fp32 gradient.
+constexpr int WM_SUB = 256;                   // points staged per sub-chunk
+constexpr int WM_LD = WM_SUB + 8;             // LDS row (bf16), 16-byte aligned rows
+
+struct WgmArgs {
+    const uint16_t *g;       // [npts][COUT] bf16
+    const float *gf;         // [npts][COUT] fp32 (bias)
+    const uint16_t *in;      // [npts][CIN] bf16
+    float *part;             // [nchunk][27][COUT*CIN]
+    float *partb;            // [nchunk][COUT]
+    int nx, ny, nz1;
+    long npts;
+};
+
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) k_wgrad_mfma(WgmArgs a) {
+    constexpr int TCO = COUT / 16, TCI = CIN / 16, NT = TCO * TCI;   // 16 x 16 tiles (<= 4)
+    __shared__ __attribute__((aligned(16))) uint16_t gT[COUT][WM_LD];
+    __shared__ __attribute__((aligned(16))) uint16_t iT[CIN][WM_LD];
+    const int t = blockIdx.y;
+    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tco = (wave % NT) / TCI, tci = (wave % NT) % TCI;
+    f4v acc = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+    float bacc = 0.0f;
+    const long base0 = (long)blockIdx.x * WG_CHUNK;
+    for (int sub = 0; sub < WG_CHUNK; sub += WM_SUB) {
+        const long base = base0 + sub;
+        for (int e = tid; e < WM_SUB * COUT; e += 256) {
+            const int i = e / COUT, c = e % COUT;
+            const long p = base + i;
+            gT[c][i] = p < a.npts ? a.g[(size_t)p * COUT + c] : (uint16_t)0;
+        }
+        for (int e = tid; e < WM_SUB * CIN; e += 256) {
+            const int i = e / CIN, c = e % CIN;
+            const long p = base + i;
+            uint16_t v = 0;
+            if (p < a.npts) {
+                const int z = (int)(p % a.nz1) + dz;
+                const long r = p / a.nz1;
+                int y = (int)(r % a.ny) + dy, x = (int)(r / a.ny) + dx;
+                x = x < 0 ? x + a.nx : (x >= a.nx ? x - a.nx : x);
+                y = y < 0 ? y + a.ny : (y >= a.ny ? y - a.ny : y);
+                if (z >= 0 && z < a.nz1) v = a.in[(((size_t)x * a.ny + y) * a.nz1 + z) * CIN + c];
+            }
+            iT[c][i] = v;
+        }
+        if (t == 13 && tid < COUT)
+            for (int i = 0; i < WM_SUB; ++i) {
+                const long p = base + i;
+                if (p < a.npts) bacc += a.gf[(size_t)p * COUT + tid];
+            }
+        __syncthreads();
+        if (wave < NT) {
+            const int co = tco * 16 + (lane & 15), ci = tci * 16 + (lane & 15), k8 = 8 * (lane >> 4);
+#pragma unroll 4
+            for (int pk = 0; pk < WM_SUB; pk += 32) {
+                const b8v af = *reinterpret_cast<const b8v *>(&gT[co][pk + k8]);
+                const b8v bf = *reinterpret_cast<const b8v *>(&iT[ci][pk + k8]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D row 4*(lane/16)+v = output channel, column lane%16 = input channel
+    if (wave < NT) {
+        float *out = a.part + ((size_t)blockIdx.x * 27 + t) * (COUT * CIN);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int co = tco * 16 + 4 * (lane >> 4) + v, ci = tci * 16 + (lane & 15);
+            out[co * CIN + ci] = acc[v];
+        }
+    }
+    if (t == 13 && tid < COUT) a.partb[(size_t)blockIdx.x * COUT + tid] = bacc;
+}
+
 // grad[w] = sum over chunks (fixed order: deterministic)
 __global__ void __launch_bounds__(256) k_wgrad_reduce(const float *part, const float *partb, int nchunk, int nw,
                                                       int cout, float *gw, float *gb) {
@@ -357,6 +441,7 @@ struct ps3d_cnn {
     float *part = nullptr, *partb = nullptr, *lpart = nullptr, *loss = nullptr, *loss_acc = nullptr;
     int nchunk = 0, nlblk = 0;
     long adam_k = 0;
+    bool wg_mfma = true;                            // weight gradient on the matrix cores
     // device dataset
     uint16_t *dx = nullptr;
     float *dt = nullptr;
@@ -404,18 +489,34 @@ void conv(const ps3d_cnn *c, const uint16_t *in, const uint16_t *w, const float 
     hipLaunchKernelGGL((k_conv<CIN, COUT, MODE>), dim3(conv_blocks(c->nx, c->ny, c->nz1)), dim3(256), 0, s, a);
 }
 
+// weight and bias gradient of one layer: g the fp32 output gradient, gb its bf16
+// copy (the matrix-core path; nullptr: the fp32 path)
 template <int CIN, int COUT>
-void wgrad(const ps3d_cnn *c, const float *g, const uint16_t *in, int layer, hipStream_t s) {
-    WgArgs a;
-    a.g = g;
-    a.in = in;
-    a.part = c->part;
-    a.partb = c->partb;
-    a.nx = c->nx;
-    a.ny = c->ny;
-    a.nz1 = c->nz1;
-    a.npts = c->npts;
-    hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(c->nchunk, 27), dim3(256), 0, s, a);
+void wgrad(const ps3d_cnn *c, const float *g, const uint16_t *gb, const uint16_t *in, int layer, hipStream_t s) {
+    if (gb && c->wg_mfma) {
+        WgmArgs m;
+        m.g = gb;
+        m.gf = g;
+        m.in = in;
+        m.part = c->part;
+        m.partb = c->partb;
+        m.nx = c->nx;
+        m.ny = c->ny;
+        m.nz1 = c->nz1;
+        m.npts = c->npts;
+        hipLaunchKernelGGL((k_wgrad_mfma<CIN, COUT>), dim3(c->nchunk, 27), dim3(256), 0, s, m);
+    } else {
+        WgArgs a;
+        a.g = g;
+        a.in = in;
+        a.part = c->part;
+        a.partb = c->partb;
+        a.nx = c->nx;
+        a.ny = c->ny;
+        a.nz1 = c->nz1;
+        a.npts = c->npts;
+        hipLaunchKernelGGL((k_wgrad<CIN, COUT>), dim3(c->nchunk, 27), dim3(256), 0, s, a);
+    }
     const int nw = 27 * COUT * CIN;
     hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(nw + COUT, 256)), dim3(256), 0, s, (const float *)c->part,
                        (const float *)c->partb, c->nchunk, nw, COUT, c->gw + c->woff[layer], c->gw + c->boff[layer]);
@@ -434,13 +535,13 @@ void backward(ps3d_cnn *c, const uint16_t *x, const float *target, bool want_gx,
                        c->lpart);
     hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1), 0, s, (const float *)c->lpart, c->nlblk,
                        1.0f / (3.0f * (float)c->npts), c->loss, c->loss_acc);
-    wgrad<32, 16>(c, c->gy, c->act[2], 3, s);
+    wgrad<32, 16>(c, c->gy, c->gyb, c->act[2], 3, s);
     conv<16, 32, CM_DG>(c, c->gyb, c->wt[3], nullptr, c->pre[2], c->gpA, c->gbA, s);
-    wgrad<32, 32>(c, c->gpA, c->act[1], 2, s);
+    wgrad<32, 32>(c, c->gpA, c->gbA, c->act[1], 2, s);
     conv<32, 32, CM_DG>(c, c->gbA, c->wt[2], nullptr, c->pre[1], c->gpB, c->gbB, s);
-    wgrad<32, 32>(c, c->gpB, c->act[0], 1, s);
+    wgrad<32, 32>(c, c->gpB, c->gbB, c->act[0], 1, s);
     conv<32, 32, CM_DG>(c, c->gbB, c->wt[1], nullptr, c->pre[0], c->gpA, c->gbA, s);
-    wgrad<16, 32>(c, c->gpA, x, 0, s);
+    wgrad<16, 32>(c, c->gpA, c->gbA, x, 0, s);
     if (want_gx) conv<32, 16, CM_DX>(c, c->gbA, c->wt[0], nullptr, nullptr, c->gx, nullptr, s);
 }
 
@@ -527,6 +628,8 @@ int ps3d_cnn_create(int nx, int ny, int nz, ps3d_cnn **out) {
     }
     c->nparam = o;
     c->nchunk = (int)((c->npts + WG_CHUNK - 1) / WG_CHUNK);
+    // PS3D_CNN_WGRAD=fp32 keeps the fp32 weight-gradient reduction (for comparison)
+    if (const char *ev = std::getenv("PS3D_CNN_WGRAD")) c->wg_mfma = std::string(ev) != "fp32";
     c->nlblk = (int)std::min<long>((c->npts + 255) / 256, 1024);
     const size_t P = (size_t)c->npts;
     auto fl = [&](float **p, size_t n) { return hipMalloc((void **)p, n * sizeof(float)); };

@@ -101,23 +101,31 @@ PS3D_CPLAN8(1536, 8, 8, 8, 3)
 PS3D_CPLAN8(2048, 8, 8, 8, 4)
 #undef PS3D_CPLAN8
 
+// LDS slot of line element x.  SKEW: one pad slot after every 16 elements, so that
+// a pass writing with a stride of R = 4, 8 or 16 elements (the first Stockham pass)
+// spreads a wave's 16-byte stores over the LDS banks instead of one bank group.
+template <bool SKEW>
+__host__ __device__ constexpr int lslot(int x) {
+    return SKEW ? x + (x >> 4) : x;
+}
+
 // Element x of the complex line whose inverse FFT gives two real lines from their
 // Hermitian real spectra (stafft.f90:55-58): slot k < n/2 holds the real parts
 // (a(k), b(k)) and slot n-k the imaginary parts; scaled by hs.
-template <int N>
+template <int N, bool SKEW = false>
 __host__ __device__ inline double2 herm_in(const double2 *s, int x, double hs) {
     constexpr int H = N / 2;
     if (x == 0 || x == H) {
-        const double2 v = s[x];
+        const double2 v = s[lslot<SKEW>(x)];
         return make_double2(hs * v.x, hs * v.y);
     }
-    const double2 a = s[x < H ? x : N - x], b = s[x < H ? N - x : x];   // real, imaginary slots
+    const double2 a = s[lslot<SKEW>(x < H ? x : N - x)], b = s[lslot<SKEW>(x < H ? N - x : x)];   // real, imaginary slots
     return x < H ? make_double2(hs * (a.x - b.y), hs * (b.x + a.y)) : make_double2(hs * (a.x + b.y), hs * (a.y - b.x));
 }
 
 // One Stockham pass with compile-time length, radix and span.  HERM (first pass
 // only): the input lines are Hermitian real spectra, combined on the fly.
-template <int N, int R, int P, bool HERM = false>
+template <int N, int R, int P, bool HERM = false, bool SKEW = false>
 __host__ __device__ inline void cpass(const double2 *src, double2 *dst, int nlines, int ls, const double2 *tw, int dir,
                                       int e0, int de, double hs = 1.0) {
     static_assert(!HERM || P == 1, "Hermitian input only in the first pass");
@@ -132,7 +140,7 @@ __host__ __device__ inline void cpass(const double2 *src, double2 *dst, int nlin
         double2 u[R];
 #pragma unroll
         for (int t = 0; t < R; ++t) {
-            double2 v = HERM ? herm_in<N>(s, i + t * m, hs) : s[i + t * m];
+            double2 v = HERM ? herm_in<N, SKEW>(s, i + t * m, hs) : s[lslot<SKEW>(i + t * m)];
             if (P > 1 && t > 0) {
                 const double2 w = tw[k * t * tstep];
                 v = dir < 0 ? cmul(v, w) : cmulc(v, w);
@@ -142,7 +150,7 @@ __host__ __device__ inline void cpass(const double2 *src, double2 *dst, int nlin
         bfly<R>(u, dir);
         const int j = (i - k) * R + k;
 #pragma unroll
-        for (int t = 0; t < R; ++t) d[j + t * P] = u[t];
+        for (int t = 0; t < R; ++t) d[lslot<SKEW>(j + t * P)] = u[t];
     }
 }
 
@@ -151,27 +159,27 @@ __host__ __device__ inline void cpass(const double2 *src, double2 *dst, int nlin
 // with b; tw[m] = exp(-2 pi i m/N).  All threads of the block call it.
 // HERM: the input lines hold Hermitian real spectra (scaled by hs on the way in),
 // which saves the separate pass that would form the complex lines.
-template <int N, typename C = CPlan<N>, bool HERM = false>
+template <int N, typename C = CPlan<N>, bool HERM = false, bool SKEW = false>
 __device__ inline double2 *cfft_lds(double2 *a, double2 *b, int nlines, int ls, const double2 *__restrict__ tw,
                                     int dir, double hs = 1.0) {
     static_assert(C::ok, "no compile-time plan for this length");
     const int tid = threadIdx.x, nthr = blockDim.x;
-    cpass<N, C::r0, 1, HERM>(a, b, nlines, ls, tw, dir, tid, nthr, hs);
+    cpass<N, C::r0, 1, HERM, SKEW>(a, b, nlines, ls, tw, dir, tid, nthr, hs);
     __syncthreads();
     if constexpr (C::r1 == 0) {
         return b;
     } else {
-        cpass<N, C::r1, C::r0>(b, a, nlines, ls, tw, dir, tid, nthr);
+        cpass<N, C::r1, C::r0, false, SKEW>(b, a, nlines, ls, tw, dir, tid, nthr);
         __syncthreads();
         if constexpr (C::r2 == 0) {
             return a;
         } else {
-            cpass<N, C::r2, C::r0 * C::r1>(a, b, nlines, ls, tw, dir, tid, nthr);
+            cpass<N, C::r2, C::r0 * C::r1, false, SKEW>(a, b, nlines, ls, tw, dir, tid, nthr);
             __syncthreads();
             if constexpr (C::r3 == 0) {
                 return b;
             } else {
-                cpass<N, C::r3, C::r0 * C::r1 * C::r2>(b, a, nlines, ls, tw, dir, tid, nthr);
+                cpass<N, C::r3, C::r0 * C::r1 * C::r2, false, SKEW>(b, a, nlines, ls, tw, dir, tid, nthr);
                 __syncthreads();
                 return a;
             }

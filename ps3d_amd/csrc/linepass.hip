@@ -18,7 +18,14 @@ struct Tile {
     int n, ZC, nl, LS, zce, z0, other;
     long base;
     const double2 *twl;   // LDS copy of the twiddles (compile-time lengths)
+    bool skew;            // LDS lines in the skewed slot layout (lslot, compile-time lengths)
+    // LDS slot of element x of a line
+    __device__ int sl(int x) const { return skew ? x + (x >> 4) : x; }
 };
+
+// LDS line stride (complex elements) of the skewed layout: n slots, n/16 pads, one
+// more so that consecutive lines start four banks apart
+__host__ __device__ constexpr int line_ls(int n) { return n + (n >> 4) + 1; }
 
 // z depth of a tile for a line length: n*ZC <= 4096 doubles, 2 <= ZC <= 16, even
 __host__ __device__ constexpr int zc_for(int n) {
@@ -39,9 +46,9 @@ __device__ inline double2 *lfft(const Tile &t, const LineArgs &a, double2 *A, do
     if constexpr (N == 0)
         return fft_lds(A, B, t.nl, t.LS, a.P, a.tw, dir);
     else if constexpr (REG && CPlan8<N>::ok)
-        return cfft_lds<N, CPlan8<N>>(A, B, t.nl, t.LS, t.twl, dir);
+        return cfft_lds<N, CPlan8<N>, false, true>(A, B, t.nl, t.LS, t.twl, dir);
     else
-        return cfft_lds<N>(A, B, t.nl, t.LS, t.twl, dir);
+        return cfft_lds<N, CPlan<N>, false, true>(A, B, t.nl, t.LS, t.twl, dir);
 }
 
 // Tile geometry fixed at compile time for a built line length: the index arithmetic
@@ -50,7 +57,8 @@ __device__ inline void fix_tile(Tile &t, int n, int zc) {
     t.n = n;
     t.ZC = zc;
     t.nl = zc / 2;
-    t.LS = n + 1;
+    t.LS = line_ls(n);
+    t.skew = true;
     const int rem = t.zce;
     t.zce = rem < zc ? rem : zc;
 }
@@ -71,6 +79,7 @@ __device__ inline Tile make_tile(const LineArgs &a) {
     t.ZC = a.ZC;
     t.nl = a.ZC / 2;
     t.LS = a.n + 1;
+    t.skew = false;
     const int ti = tile_index();
     const int zt = ti % a.nzt;
     t.other = ti / a.nzt;
@@ -105,7 +114,7 @@ __device__ inline void tile_load(const Tile &t, const LineArgs &a, const double 
     const int tot = t.n * t.nl;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int p = e % t.nl, j = e / t.nl;
-        buf[(size_t)p * t.LS + j] = gpair(t, a, src, j, p);
+        buf[(size_t)p * t.LS + t.sl(j)] = gpair(t, a, src, j, p);
     }
 }
 
@@ -117,7 +126,7 @@ __device__ inline void tile_store(const Tile &t, const LineArgs &a, double *__re
         const int tot = t.n * t.nl;
         for (int e = threadIdx.x; e < tot; e += blockDim.x) {
             const int p = e % t.nl, j = e / t.nl;
-            const double2 v = buf[(size_t)p * t.LS + j];
+            const double2 v = buf[(size_t)p * t.LS + t.sl(j)];
             spair(t, a, dst, j, p, make_double2(v.x * scale, v.y * scale));
         }
         return;
@@ -126,7 +135,7 @@ __device__ inline void tile_store(const Tile &t, const LineArgs &a, double *__re
     const int tot = t.n * t.ZC;
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int zz = e % t.ZC, j = e / t.ZC;
-        if (zz < t.zce) dst[t.base + (long)j * a.estride + zz] = d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] * scale;
+        if (zz < t.zce) dst[t.base + (long)j * a.estride + zz] = d[((size_t)(zz >> 1) * t.LS + t.sl(j)) * 2 + (zz & 1)] * scale;
     }
 }
 
@@ -141,21 +150,22 @@ __device__ inline double2 *tile_inverse(const Tile &t, const LineArgs &a, double
     // faster with the separate pass)
     if constexpr (N >= 128) {
         if constexpr (REG && CPlan8<N>::ok)
-            return cfft_lds<N, CPlan8<N>, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
+            return cfft_lds<N, CPlan8<N>, true, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
         else
-            return cfft_lds<N, CPlan<N>, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
+            return cfft_lds<N, CPlan<N>, true, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
     }
     const int tot = t.nl * (h + 1);
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int line = e / (h + 1), k = e % (h + 1);
         double2 *L = A + (size_t)line * t.LS;
+        const int sk_ = t.sl(k), snk = t.sl(n - k);
         if (k == 0 || k == h) {
-            const double2 v = L[k];
-            L[k] = make_double2(sn * v.x, sn * v.y);
+            const double2 v = L[sk_];
+            L[sk_] = make_double2(sn * v.x, sn * v.y);
         } else {
-            const double2 ck = L[k], sk = L[n - k];   // (a(k), b(k)), (a(n-k), b(n-k))
-            L[k] = make_double2(sn * (ck.x - sk.y), sn * (sk.x + ck.y));
-            L[n - k] = make_double2(sn * (ck.x + sk.y), sn * (-sk.x + ck.y));
+            const double2 ck = L[sk_], sk = L[snk];   // (a(k), b(k)), (a(n-k), b(n-k))
+            L[sk_] = make_double2(sn * (ck.x - sk.y), sn * (sk.x + ck.y));
+            L[snk] = make_double2(sn * (ck.x + sk.y), sn * (-sk.x + ck.y));
         }
     }
     __syncthreads();
@@ -173,11 +183,11 @@ __device__ inline double2 *tile_forward(const Tile &t, const LineArgs &a, double
         const int line = e / (h + 1), k = e % (h + 1);
         double2 *L = R + (size_t)line * t.LS;
         const int nk = (n - k) % n;
-        const double2 zk = L[k], znk = L[nk];
+        const double2 zk = L[t.sl(k)], znk = L[t.sl(nk)];
         const double Ar = 0.5 * (zk.x + znk.x), Ai = 0.5 * (zk.y - znk.y);
         const double Br = 0.5 * (zk.y + znk.y), Bi = -0.5 * (zk.x - znk.x);
-        L[k] = make_double2(Ar * sc, Br * sc);
-        if (k != 0 && k != h) L[nk] = make_double2(Ai * sc, Bi * sc);
+        L[t.sl(k)] = make_double2(Ar * sc, Br * sc);
+        if (k != 0 && k != h) L[t.sl(nk)] = make_double2(Ai * sc, Bi * sc);
     }
     __syncthreads();
     return R;
@@ -196,7 +206,7 @@ __device__ inline void tile_store_fwd(const Tile &t, const LineArgs &a, double *
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
         const int p = e % t.nl, j = e / t.nl;
         const double2 *L = R + (size_t)p * t.LS;
-        const double2 zj = L[j], zm = L[j == 0 ? 0 : N - j];
+        const double2 zj = L[t.sl(j)], zm = L[t.sl(j == 0 ? 0 : N - j)];
         double2 v;
         if (j <= H)
             v = make_double2(sc * (zj.x + zm.x), sc * (zj.y + zm.y));
@@ -215,8 +225,8 @@ __device__ inline void tile_diff(const Tile &t, const LineArgs &a, const double2
         const int line = e / n, j = e % n;
         double f = (j >= h + 1) ? a.rk[j] : -a.rk[j];
         if (j == 0 || j == h) f = 0.0;
-        const double2 v = src[(size_t)line * t.LS + (n - j) % n];
-        dst[(size_t)line * t.LS + j] = make_double2(f * v.x, f * v.y);
+        const double2 v = src[(size_t)line * t.LS + t.sl((n - j) % n)];
+        dst[(size_t)line * t.LS + t.sl(j)] = make_double2(f * v.x, f * v.y);
     }
     __syncthreads();
 }
@@ -229,7 +239,7 @@ __device__ inline void to_regs(const Tile &t, const double2 *buf, double2 (&r)[k
         const int e = threadIdx.x + k * blockDim.x;
         if (e < tot) {
             const int line = e / t.n, j = e % t.n;
-            const double2 v = buf[(size_t)line * t.LS + j];
+            const double2 v = buf[(size_t)line * t.LS + t.sl(j)];
             r[k] = make_double2(v.x * scale, v.y * scale);
         } else {
             r[k] = make_double2(0.0, 0.0);
@@ -243,7 +253,7 @@ __device__ inline void from_regs(const Tile &t, double2 *buf, const double2 (&r)
         const int e = threadIdx.x + k * blockDim.x;
         if (e < tot) {
             const int line = e / t.n, j = e % t.n;
-            buf[(size_t)line * t.LS + j] = r[k];
+            buf[(size_t)line * t.LS + t.sl(j)] = r[k];
         }
     }
 }
@@ -348,7 +358,7 @@ __device__ inline void pf_commit(const Pf<N, T> &p, const Tile &t, double2 *A) {
 #pragma unroll
     for (int k = 0; k < Pf<N, T>::PER; ++k) {
         const int e = threadIdx.x + k * T, zz = e % ZC, j = e / ZC;
-        if (j < t.n) d[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = p.pre[k];
+        if (j < t.n) d[((size_t)(zz >> 1) * t.LS + t.sl(j)) * 2 + (zz & 1)] = p.pre[k];
     }
 }
 // field_phys on the prefetched tile (spectral input; diff: x/y derivative first),
@@ -539,10 +549,10 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
                 const int e = threadIdx.x + k * 256;
                 if constexpr (PAIR) {
                     const int q = e % NL, j = e / NL;
-                    if (j < N) A[(size_t)q * t.LS + j] = pre[k];
+                    if (j < N) A[(size_t)q * t.LS + t.sl(j)] = pre[k];
                 } else {
                     const int zz = e % ZC, j = e / ZC;
-                    if (j < N) reinterpret_cast<double *>(A)[((size_t)(zz >> 1) * t.LS + j) * 2 + (zz & 1)] = pre1[k];
+                    if (j < N) reinterpret_cast<double *>(A)[((size_t)(zz >> 1) * t.LS + t.sl(j)) * 2 + (zz & 1)] = pre1[k];
                 }
             }
             __syncthreads();
@@ -949,7 +959,7 @@ k_line_reg(LineArgs a) {
 }
 
 size_t line_lds_bytes(int n, int ZC) {
-    const size_t tb = (size_t)(ZC / 2) * (n + 1) * sizeof(double2);
+    const size_t tb = (size_t)(ZC / 2) * line_ls(n) * sizeof(double2);
     size_t need = 2 * tb + (size_t)n * sizeof(double2);
     const size_t red = kLineThreads * sizeof(double);
     return need > red ? need : red;
@@ -997,7 +1007,7 @@ static void launch_line_t(const LineArgs &a, hipStream_t stream) {
     const int nb = a.bt.nb > 1 ? a.bt.nb : 1;
     size_t lds = line_lds_bytes(a.n, a.ZC);
     if ((a.mode == LM_XMAIN || a.mode == LM_XADAPT) && N > 0)
-        lds += (size_t)2 * (a.ZC / 2) * (a.n + 1) * sizeof(double2);   // field pairs
+        lds += (size_t)2 * (a.ZC / 2) * line_ls(a.n) * sizeof(double2);   // field pairs
     if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF)
         hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks, nb), dim3(256), lds, stream, a);
     else if (a.rthreads == 512 && N > 0)

@@ -65,6 +65,32 @@ def test_gradients_match_oracle(CL):
     net.close()
 
 
+def test_weight_gradient_on_matrix_cores_matches_fp32_path(CL, monkeypatch):
+    """The bf16 matrix-core weight gradient against the fp32 reduction
+    (PS3D_CNN_WGRAD=fp32, read when a net is created) on a grid whose point count
+    leaves a partial last chunk: per-layer direction and magnitude."""
+    shape = (12, 10, 20)
+    monkeypatch.setenv("PS3D_CNN_WGRAD", "fp32")
+    ref_net, params, x, t = setup_net(CL, shape, 5)
+    monkeypatch.delenv("PS3D_CNN_WGRAD")
+    net, _, _, _ = setup_net(CL, shape, 5)
+    _, g_ref, _ = ref_net.grad(x.reshape(-1, 6), t.reshape(-1, 3))
+    _, g, _ = net.grad(x.reshape(-1, 6), t.reshape(-1, 3))
+    o = 0
+    for cin, cout in N.LAYERS:
+        nw = 27 * cout * cin
+        a, b = g[o:o + nw], g_ref[o:o + nw]
+        cos = np.dot(a, b) / np.linalg.norm(a) / np.linalg.norm(b)
+        print("layer", cin, cout, "cos", cos, "norm ratio", np.linalg.norm(a) / np.linalg.norm(b), flush=True)
+        assert cos > 0.9999
+        assert abs(np.linalg.norm(a) / np.linalg.norm(b) - 1.0) < 1e-2
+        ab, bb = g[o + nw:o + nw + cout], g_ref[o + nw:o + nw + cout]
+        assert np.abs(ab - bb).max() <= 1e-4 * np.abs(bb).max()
+        o += nw + cout
+    net.close()
+    ref_net.close()
+
+
 def test_adam_step_matches_oracle(CL):
     net, params, x, t = setup_net(CL, (8, 6, 5), 3)
     w0 = net.get_weights().astype(np.float64)
