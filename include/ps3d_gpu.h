@@ -197,6 +197,20 @@ int ps3d_gpu_derived(ps3d_handle *h);
  * The partials and the device copy of the record are allocated on first use. */
 int ps3d_gpu_field_stats(ps3d_handle *h, const double *diag, double *stats);
 
+/* ps3d_gpu_field_stats without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  stats_dev is DEVICE memory of
+ * PS3D_NSTAT doubles; the record is written there on the handle's stream (read it
+ * after ps3d_gpu_sync).  The per-block partials go to one array the library keeps
+ * per GPU (a fixed 1024 blocks x 20 doubles = 160 KiB, held until the process
+ * ends); a grid whose partition would need more returns PS3D_E_UNSUPPORTED.  Needs
+ * ps3d_gpu_vor2vel and then ps3d_gpu_derived first (PS3D_E_STATE otherwise), since
+ * computing the divergence here would allocate.  Every call on one GPU shares
+ * that array: calls on one GPU that overlap in time (different handles on
+ * different streams) race on it and must be serialised by the caller. */
+int ps3d_gpu_field_stats_noalloc(ps3d_handle *h, const double *diag, double *stats_dev);
+
+
 /* Hyperdiffusion coefficients: vvisc of init_diffusion (inversion_utils.f90:157-184)
  * and the prefactor of the last adapt (get_diffusion_pre_factor, advance.f90:381-410);
  * column (k, l) decays at the rate pref * visc * (k^2 + l^2)^nnu.  Either pointer may

@@ -252,6 +252,7 @@ enum {
     ST_NSUM = ST_MIN0 - ST_S0, ST_NMIN = ST_MAX0 - ST_MIN0, ST_NMAX = ST_NP - ST_MAX0
 };
 constexpr int kStatsMaxBlocks = 1024;   // blocks of the partials pass (any grid)
+constexpr int kMaxDevices = 64;
 
 struct StatsArgs {
     int nx, ny, nz, zs;
@@ -269,6 +270,9 @@ struct StatsFin {
 };
 int stats_blocks(int nx, int ny);
 hipError_t launch_stats(const StatsArgs &a, const StatsFin &f, hipStream_t stream);
+// device address of the library's own partials array (kStatsMaxBlocks * ST_NP
+// doubles) on device dev
+hipError_t stats_scratch(int dev, double **ptr);
 
 // closure hooks (closure.hip)
 bool cnn_matches(const ps3d_cnn *c, int nx, int ny, int nz);

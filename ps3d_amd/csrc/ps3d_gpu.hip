@@ -374,6 +374,10 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     HIPCHK(ps3d::configure_zpass());
     HIPCHK(ps3d::configure_zwave());
     HIPCHK(ps3d::configure_linepass());
+    {   // device address of the allocation-free partials array (once per device)
+        double *scr = nullptr;
+        HIPCHK(ps3d::stats_scratch(h->device, &scr));
+    }
 
     const size_t nb = (size_t)nbatch;
     // per-tile partial sums sized for the shallowest tiles (2 levels), so that any
@@ -1330,6 +1334,18 @@ int ps3d_gpu_field_stats(ps3d_handle *h, const double *diag, double *stats) {
     HIPCHK(hipMemcpyAsync(stats, h->stats_out, sizeof(double) * PS3D_NSTAT, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
+}
+
+int ps3d_gpu_field_stats_noalloc(ps3d_handle *h, const double *diag, double *stats_dev) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
+    if (!h || !stats_dev) return fail(PS3D_E_ARG, "null argument");
+    if (!h->have_phys || !h->derived_fresh)
+        return fail(PS3D_E_STATE, "field_stats_noalloc needs vor2vel and then ps3d_gpu_derived first");
+    if ((long)stats_blocks(h->g.nx, h->g.ny) * ST_NP > (long)kStatsMaxBlocks * ST_NP)
+        return fail(PS3D_E_UNSUPPORTED, "field_stats_noalloc: grid needs more partials than the library's array holds");
+    double *scr = nullptr;
+    HIPCHK(stats_scratch(h->device, &scr));
+    return field_stats_launch(h, diag, scr, stats_dev);
 }
 
 int ps3d_gpu_get(ps3d_handle *h, int which, double *host) {

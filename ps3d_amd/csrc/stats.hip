@@ -9,6 +9,8 @@
 #include "kernels.hpp"
 #include "ps3d_gpu.h"
 
+#include <mutex>
+
 namespace ps3d {
 
 namespace {
@@ -186,6 +188,37 @@ hipError_t launch_stats(const StatsArgs &a, const StatsFin &f, hipStream_t strea
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_stats_fin, dim3(1), dim3(64), 0, stream, f);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free partials: one array per device, found once per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[kStatsMaxBlocks * ST_NP];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on device dev (the array's host-side name is not
+// a device address; each device holds its own copy).  Looked up once per device,
+// with that device current, and kept.
+hipError_t stats_scratch(int dev, double **ptr) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *ptr = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 }  // namespace ps3d
