@@ -42,6 +42,9 @@ def parse():
                     help="beltrami: 4 seeded Beltrami modes + noise; random: random-phase isotropic "
                          "E(k) ~ k^4 exp(-2 (k/4)^2) made on 128^3 and zero-padded to n^3 on the GPU")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-instance ensemble measurement")
+    ap.add_argument("--no-closure", action="store_true", help="skip the closure-64^3 vs plain-256^3 measurement")
+    ap.add_argument("--closure-seed", type=int, default=9001, help="held-out seed of the closure measurement")
+    ap.add_argument("--closure-steps", type=int, default=1000, help="fine 256^3 steps of the closure measurement")
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--separate-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -111,6 +114,7 @@ def ensemble_bench(G, steps, sep_steps):
             i += 1
     shapes = ens.tune(steps=2)
     line_threads = ens.line_choices()
+    fft_sched = ens.fft_schedules()
     ens.advance(2)                      # builds the graph
     t0 = time.perf_counter()
     ev_ms = ens.advance(steps, graph=True)
@@ -138,11 +142,42 @@ def ensemble_bench(G, steps, sep_steps):
         "grid_point_updates_per_s": npts * steps / el,
         "z_shapes": {str(n): c for n, c in zip(ENSEMBLE_SIZES, shapes)},
         "line_threads": {str(n): list(c) for n, c in zip(ENSEMBLE_SIZES, line_threads)},
+        "fft_schedule_xy": {str(n): list(c) for n, c in zip(ENSEMBLE_SIZES, fft_sched)},
         "step_roofline": {"bound": "hbm", "algo_bytes_per_step": b_step, "achieved": b_step * steps / el / 1e9,
                           "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": b_step * steps / el / PEAK_HBM},
         "separate": {"steps": sep_steps, "ms_per_step": 1e3 * el_sep / sep_steps,
                      "grid_point_updates_per_s": npts * sep_steps / el_sep},
         "speedup_vs_separate": (npts * steps / el) / (npts * sep_steps / el_sep),
+    }
+
+
+CLOSURE_WEIGHTS = os.path.join(HERE, "ps3d_amd", "data", "closure64.npz")
+
+
+def closure_bench(G, seed, fine_steps, timing_steps):
+    """The closure half of the metric on one held-out seed: the trained 64^3 net
+    (2000 snapshots of twenty 256^3 runs) against the plain 256^3 run from the same
+    random-phase initial state (k0 = 5) over `fine_steps` fine steps, and the plain
+    64^3 run at the prediss tuned without the net.  Errors at the end against the
+    fine run truncated to the modes a 64^3 step passes: the genspec spectrum below the 2/3 cutoff
+    (relative L2) and the enstrophy (relative)."""
+    from ps3d_amd import closure_run as CR
+    net, gain, pd_c, meta = CR.load_net(CLOSURE_WEIGHTS, 64)
+    pd_p = meta.get("coarse_prediss_plain_tuned", pd_c)
+    cfg = [("no_closure_tuned", pd_p, None), ("closure", pd_c, gain)]
+    ev = CR.evaluate(256, 64, net, [seed], fine_steps, cfg)[0]
+    st = CR.step_timing(256, 64, net, steps=timing_steps)
+    net.close()
+    c = ev["closure"]
+    return {
+        "weights": os.path.relpath(CLOSURE_WEIGHTS, HERE), "training": meta, "seed": seed,
+        "fine_steps": fine_steps, "t_end": ev["t_end"], "gain": gain, "coarse_prediss": pd_c,
+        "fine_s": ev["fine_s"], "closure_s": c["s"], "closure_steps": c["steps"],
+        "speedup_per_time": ev["fine_s"] / c["s"],
+        "ms_per_step": st, "speedup_per_step": st["fine"] / st["coarse_closure"],
+        "spectrum_error": {k: ev[k]["spectrum_error"] for k, _, _ in cfg},
+        "enstrophy_error": {k: ev[k]["enstrophy_error"] for k, _, _ in cfg},
+        "target": "closure 64^3 >= 20x faster than plain 256^3 at <= 5% spectrum error",
     }
 
 
@@ -318,6 +353,11 @@ def main():
             out["ensemble"] = ensemble_bench(G, a.ensemble_steps, a.separate_steps)
         except Exception as exc:       # reported, never silently dropped
             out["ensemble"] = {"error": repr(exc)}
+    if not a.no_closure:
+        try:
+            out["closure"] = closure_bench(G, a.closure_seed, a.closure_steps, 20)
+        except Exception as exc:       # reported, never silently dropped
+            out["closure"] = {"error": repr(exc)}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(vor, n, a.cpu_steps, a.cpu_threads)
     print(json.dumps(out))

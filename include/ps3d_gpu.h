@@ -130,8 +130,8 @@ const char *ps3d_gpu_last_error(void);
 const char *ps3d_gpu_version(void);
 
 /* = init_inversion + initialise_fft (inversion_utils.f90:222-372, sta3dfft.f90:53-110)
- * on the current HIP device. Rejects lengths with a prime factor other than 2 and 3
- * (nx, ny even; nz even) with PS3D_E_FACTOR.  The handle owns a non-blocking stream;
+ * on the current HIP device. Rejects lengths with a prime factor other than 2, 3
+ * and 5 (factorisen, stafft.f90:128-187; nx, ny, nz even) with PS3D_E_FACTOR.  The handle owns a non-blocking stream;
  * its device memory is cleared and the set-up complete when this returns. */
 int ps3d_gpu_create(const ps3d_params *p, ps3d_handle **h);
 /* = finalise_inversion (inversion_utils.f90:459-480). */
@@ -196,6 +196,32 @@ int ps3d_gpu_derived(ps3d_handle *h);
  * or nan when it is 0); bfmax, rolling_mean_bfmax and ri_min are 0 (no buoyancy).
  * The partials and the device copy of the record are allocated on first use. */
 int ps3d_gpu_field_stats(ps3d_handle *h, const double *diag, double *stats);
+
+/* Domain means of the derived fields of ps3d's nc_reader (plotting/nc_reader.py:
+ * 94-120) over the state of the last ps3d_gpu_vor2vel: kinetic energy as
+ * get_kinetic_energy (field_diagnostics.f90:85-122), helicity u.w (the adapt
+ * record's), the cross-helicity magnitude |u x w| with the lids weighted 1/2, and
+ * the largest |u x w|. */
+enum ps3d_moment {
+    PS3D_MO_KE = 0, PS3D_MO_HELICITY, PS3D_MO_CROSS_HELICITY, PS3D_MO_CROSS_HELICITY_MAX,
+    PS3D_NMOMENT
+};
+int ps3d_gpu_moments(ps3d_handle *h, double *out);
+
+/* = genspec (src/genspec.f90:44-124) on the GPU: the kinetic-energy spectrum of a
+ * physical velocity in shells of |K| = sqrt(kx^2 + ky^2 + kz^2).  vel is a host
+ * array of 3 volumes (as ps3d_gpu_get's PS3D_VEL), which then replaces the handle's
+ * physical velocity (the state needs ps3d_gpu_vor2vel again before it steps), or
+ * NULL for the velocity of the last ps3d_gpu_vor2vel.  Shell m = int(nint(|K|)/dk),
+ * m = 0..kmax, holds spec[m] = sum of squared coefficients * 4/3 pi dk^3
+ * ((m+1)^3 - m^3) / num[m], normalised so that sum(spec) dk = get_kinetic_energy.
+ * *nshell = kmax + 1 and *dk are always set; spec == NULL only queries them; cap
+ * (the length of spec and num) below *nshell gives PS3D_E_ARG.  num (may be NULL)
+ * receives the point count of each shell.  Uses the y-derivative volumes as scratch:
+ * ps3d_gpu_derived and ps3d_gpu_source need a new vor2vel afterwards.  Buffers are
+ * allocated on first use. */
+int ps3d_gpu_spectrum(ps3d_handle *h, const double *vel, int cap, int *nshell, double *dk, double *spec,
+                      long long *num);
 
 /* ps3d_gpu_field_stats without any allocation: no counterpart in the reference;
  * for callers that cannot allocate device memory (code under HIP graph capture,
@@ -298,7 +324,8 @@ int ps3d_gpu_ensemble_group(ps3d_ensemble *e, int g, ps3d_handle **h);
 int ps3d_gpu_ensemble_advance(ps3d_ensemble *e, int nsteps, int use_graph, double *ms);
 /* Times `steps` steps of each group under each available z launch shape and
  * keeps the fastest, then, one mode at a time, the block size of each
- * register-holding line pass (state restored afterwards).  choice[g] (may be
+ * register-holding line pass, then the FFT schedule of the transform-only y and x
+ * line passes (state restored afterwards).  choice[g] (may be
  * NULL) receives the shape code: 100..102 = one wavefront per column (register
  * budget variant), q < 100 = q quads per block of the quad-per-thread kernels. */
 int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice);
@@ -307,6 +334,11 @@ int ps3d_gpu_ensemble_set_choice(ps3d_ensemble *e, int g, int code);
  * threads[0] x products, [1] adapt-stage y, [2] x adapt (1024, 512 or 256: full,
  * half or quarter tile depth).  No counterpart in the reference. */
 int ps3d_gpu_ensemble_line_choice(ps3d_ensemble *e, int g, int *threads);
+/* FFT schedules the tuner chose for group g's transform-only line passes:
+ * sched[0] x lines, sched[1] y lines; bit 1 = radix-8-first plan (else
+ * radix-16-first), bit 2 = the Hermitian pair of the inverse formed in the first
+ * radix pass (else a pass of its own).  No counterpart in the reference. */
+int ps3d_gpu_ensemble_fft_schedule(ps3d_ensemble *e, int g, int *sched);
 int ps3d_gpu_ensemble_destroy(ps3d_ensemble *e);
 
 #ifdef __cplusplus

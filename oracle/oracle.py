@@ -264,6 +264,67 @@ class Oracle:
 
 
 # --------------------------------------------------------------------------
+# genspec's spectrum and nc_reader's derived means (numpy)
+# --------------------------------------------------------------------------
+def kinetic_energy(vel):
+    """get_kinetic_energy (src/fields/field_diagnostics.f90:85-122): half the sum
+    of |u|^2 with the lid levels weighted 1/2, over nx ny nz cells."""
+    e = np.sum(vel ** 2, axis=0)                     # (nx, ny, nz+1)
+    nx, ny, nz1 = e.shape
+    tot = 0.5 * e[:, :, 1:-1].sum() + 0.25 * (e[:, :, 0].sum() + e[:, :, -1].sum())
+    return tot / (nx * ny * (nz1 - 1))
+
+
+def moments(vel, vor):
+    """Kinetic energy, and the means of the helicity u.w and of the cross-helicity
+    magnitude |u x w| (plotting/nc_reader.py:94-120) with the lids weighted 1/2,
+    over nx ny nz cells; and the largest |u x w|."""
+    nx, ny, nz1 = vel.shape[1:]
+    w = np.ones(nz1)
+    w[0] = w[-1] = 0.5
+    hel = np.sum(vel * vor, axis=0)
+    cr = np.linalg.norm(np.cross(vel, vor, axis=0), axis=0)
+    n = nx * ny * (nz1 - 1)
+    return {"ke": kinetic_energy(vel), "helicity": float((hel * w).sum() / n),
+            "cross_helicity": float((cr * w).sum() / n), "cross_helicity_max": float(cr.max())}
+
+
+def _nint(x):
+    """Fortran nint for x >= 0: halves round away from zero (numpy's rint rounds
+    them to even)."""
+    f = np.floor(x)
+    return np.where(x - f >= 0.5, f + 1.0, f)
+
+
+def spectrum(o, vel):
+    """genspec (src/genspec.f90:44-124) with the oracle's transforms, on the grid of
+    Oracle o: (shell wavenumbers m dk, spectrum, point counts), m = 0..kmax."""
+    nx, ny, nz = o.nx, o.ny, o.nz
+    ke = kinetic_energy(vel)                                          # :46
+    s = [o.op(OP_FFTXYP2S, vel[c]) for c in range(3)]                 # :49-52
+    s[0] = o.op(OP_FFTCOSINE, s[0])                                   # :55 dct(u)
+    s[1] = o.op(OP_FFTCOSINE, s[1])                                   # :56 dct(v)
+    s[2] = o.op(OP_FFTSINE, s[2])                                     # :57 dst(w(1:nz))
+    rkx, rky, rkz = o.table(13), o.table(14), o.table(15)
+    k2 = (rkx[:, None, None] ** 2 + rky[None, :, None] ** 2) + rkz[None, None, :] ** 2
+    kmag = _nint(np.sqrt(k2)).astype(np.int64)                        # :62-68
+    kmax = int(kmag.max())                                            # :70
+    dk = kmax / np.sqrt((0.5 * nx) ** 2 + (0.5 * ny) ** 2 + float(nz) ** 2)   # :83
+    dki = 1.0 / dk
+    m = (kmag * dki).astype(np.int64)                                 # :93
+    e = s[0] ** 2 + s[1] ** 2 + s[2] ** 2
+    keep = m <= kmax       # the reference's spec(0:kmax) has no room for m > kmax
+    spec = np.bincount(m[keep], weights=e[keep], minlength=kmax + 1)[:kmax + 1]
+    num = np.bincount(m[keep], minlength=kmax + 1)[:kmax + 1]
+    pref = 4.0 / 3.0 * np.pi * (dk * dk * dk)                         # :104
+    mm = np.arange(kmax + 1, dtype=np.int64)
+    ok = num > 0
+    spec = np.where(ok, spec * pref * ((mm + 1) ** 3 - mm ** 3).astype(np.float64) / np.maximum(num, 1), 0.0)
+    spec = spec * (ke / np.sum(spec * dk))                            # :112-117
+    return mm * dk, spec, num
+
+
+# --------------------------------------------------------------------------
 # The reference's own compiled transform core (oracle/_ref)
 # --------------------------------------------------------------------------
 _ref = None

@@ -226,37 +226,12 @@ def _fold(nf, nc):
 # --------------------------------------------------------------------------
 # genspec energy spectrum (src/genspec.f90:50-124)
 # --------------------------------------------------------------------------
-def energy_spectrum(solver, ke):
-    """Spectrum of the solver's current physical velocity (after vor2vel) in
-    shells of K = sqrt(kx^2 + ky^2 + kz^2), normalised so sum(spec) dk = ke."""
-    return velocity_spectrum(solver, solver.get(G.VEL), ke)
-
-
-def velocity_spectrum(solver, vel, ke):
-    """genspec's spectrum of a physical velocity (3 volumes) on solver's grid:
-    (shell wavenumbers k dk, spectrum), k = 0 .. kmax."""
-    nx, ny, nz = solver.nx, solver.ny, solver.nz
-    s = [solver.op(G.OP_FFTXYP2S, vel[c]) for c in range(3)]
-    s[0] = solver.op(G.OP_FFTCOSINE, s[0])
-    s[1] = solver.op(G.OP_FFTCOSINE, s[1])
-    # dst(1, nz, svel(1:nz)): w(1..nz-1) sine coefficients, w(nz) set to 0, w(0) kept
-    s[2] = solver.op(G.OP_FFTSINE, s[2])
-    lx, ly, lz = solver.p.extent[0], solver.p.extent[1], solver.p.extent[2]
-    kx = 2 * np.pi / lx * np.minimum(np.arange(nx), nx - np.arange(nx))
-    ky = 2 * np.pi / ly * np.minimum(np.arange(ny), ny - np.arange(ny))
-    kz = np.pi / lz * np.arange(nz + 1)
-    kmag = np.rint(np.sqrt(kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2)).astype(int)
-    kmax = int(kmag.max())
-    dk = kmax / math.sqrt((0.5 * nx) ** 2 + (0.5 * ny) ** 2 + float(nz) ** 2)
-    m = (kmag * (1.0 / dk)).astype(int)
-    e = s[0] ** 2 + s[1] ** 2 + s[2] ** 2
-    spec = np.bincount(m.ravel(), weights=e.ravel(), minlength=kmax + 1)[:kmax + 1]
-    num = np.bincount(m.ravel(), minlength=kmax + 1)[:kmax + 1]
-    pref = 4.0 / 3.0 * np.pi * dk ** 3
-    ok = num > 0
-    spec[ok] = spec[ok] * pref * ((np.arange(kmax + 1)[ok] + 1) ** 3 - np.arange(kmax + 1)[ok] ** 3) / num[ok]
-    spec *= ke / np.sum(spec * dk)
-    return np.arange(kmax + 1) * dk, spec
+def energy_spectrum(solver):
+    """genspec's spectrum (src/genspec.f90:44-124) of the solver's current physical
+    velocity (after vor2vel), binned on the GPU: (shell wavenumbers m dk, spectrum),
+    normalised so that sum(spec) dk is the kinetic energy."""
+    k, spec, _ = solver.spectrum()
+    return k, spec
 
 
 def spectrum_error(spec, ref, kcut=None):

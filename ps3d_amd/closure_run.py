@@ -3,6 +3,7 @@ compare the coarse run with and without it against the filtered fine run.
 
   python -m ps3d_amd.closure_run [--fine 256] [--coarse 64] [--runs 20] [--snaps 100]
                                  [--eval-steps 1000] [--eval-seeds 20] [--save FILE]
+  python -m ps3d_amd.closure_run --load ps3d_amd/data/closure64.npz [--eval-seeds 20]
 
 Fine runs: random-phase isotropic initial vorticity, E(k) ~ k^4 exp(-2 (k/k0)^2)
 with k0 ~ U[3, 8] per run (seeded), made on 128^3 and zero-padded to the fine
@@ -14,9 +15,9 @@ coarse grid's prediss are chosen (and, for the comparison, the best prediss
 without closure); held-out seeds are then run for `eval-steps` fine steps and the
 coarse grids to the same physical time from the truncated initial state: without
 closure at prediss 30, without closure at its best prediss, and with the closure.
-Errors at the end against the fine run truncated to the coarse grid: the genspec
-energy spectrum below the coarse 2/3 cutoff (relative L2) and the enstrophy
-(relative).  Progress goes to stderr; the result is
+Errors at the end against the fine run truncated to the modes the coarse step
+passes (its 2/3-rule mask): the genspec energy spectrum below the coarse 2/3 cutoff
+(relative L2) and the enstrophy (relative).  Progress goes to stderr; the result is
 one JSON object on stdout; --save writes the trained net (weights, scales, gain,
 coarse prediss) for later runs.
 """
@@ -109,14 +110,15 @@ def train(net, nsamples, epochs, lr, seed):
 def spectrum_and_enstrophy(h, t):
     h.vor2vel()
     _, row = h.adapt(t)
-    return CL.energy_spectrum(h, row[KE])[1], row[EN]
+    return CL.energy_spectrum(h)[1], row[EN]
 
 
 def evaluate(nf, nc, net, seeds, nsteps, configs):
     """Per seed: the fine run for nsteps steps (to t_end), and the coarse runs (tag,
     coarse prediss, closure output gain or None for no closure) from the truncated
     initial state to t_end (their last step clipped to it by time%limit); spectrum
-    and enstrophy errors against the fine state truncated to the coarse grid."""
+    and enstrophy errors against the fine state truncated to the modes the coarse
+    step passes."""
     out = []
     kcut = int(nc / 3)   # shells below the coarse 2/3 cutoff
     base_scale = np.array(net.out_scale, dtype=np.float32)
@@ -130,6 +132,10 @@ def evaluate(nf, nc, net, seeds, nsteps, configs):
         t_end = f.t
         cref = G.Ps3dGpu(params(nc))
         G.resample(cref, f)
+        # the reference keeps only the modes a coarse step can hold (its 2/3-rule
+        # mask): the enstrophy of the modes between that cutoff and the coarse grid's
+        # Nyquist limit is out of every coarse run's reach
+        cref.set(G.SVOR, cref.get(G.SVOR) * CL.dealias_mask(cref)[None])
         spec_f, en_f = spectrum_and_enstrophy(cref, t_end)
         f.close()
         cref.close()
@@ -191,13 +197,14 @@ def save_net(path, net, gain, pd_c, meta):
 
 
 def load_net(path, n):
-    """(net on an n^3 grid with the saved weights and scales, gain, coarse prediss)."""
+    """(net on an n^3 grid with the saved weights and scales, gain, coarse prediss,
+    meta dict)."""
     z = np.load(path, allow_pickle=False)
     net = CL.ClosureNet(n, n, n)
     net.set_weights(z["weights"])
     net.in_scale = z["in_scale"].astype(np.float32)
     net.out_scale = z["out_scale"].astype(np.float32)
-    return net, float(z["gain"]), float(z["coarse_prediss"])
+    return net, float(z["gain"]), float(z["coarse_prediss"]), json.loads(str(z["meta"]))
 
 
 def main(argv=None):
@@ -226,7 +233,11 @@ def main(argv=None):
                     help="closure target: the subgrid nonlinear transfer, or also the difference of the "
                          "two grids' hyperdiffusion")
     ap.add_argument("--save", default=None, help="write the trained net here (.npz)")
+    ap.add_argument("--load", default=None,
+                    help="evaluate a saved net (.npz, with its gain and prediss choices) instead of training one")
     a = ap.parse_args(argv)
+    if a.load:
+        return evaluate_saved(a)
     net = CL.ClosureNet(a.coarse, a.coarse, a.coarse)
     net.set_weights(CL.init_weights(0))
     t0 = time.perf_counter()
@@ -251,7 +262,10 @@ def main(argv=None):
         _, pd_c, gain = min((c for c in cfg if c[2] is not None), key=lambda c: err[c[0]])
         _, pd_p, _ = min((c for c in cfg if c[2] is None), key=lambda c: err[c[0]])
     meta = {"fine": a.fine, "coarse": a.coarse, "snapshots": ns, "runs": a.runs, "every": a.every,
-            "epochs": a.epochs, "target": a.target, "dealias_target": a.dealias_target}
+            "epochs": a.epochs, "target": a.target, "dealias_target": a.dealias_target,
+            "coarse_prediss_plain_tuned": pd_p,
+            "training_seeds": "fine runs 500..%d, k0 ~ U[3, 8] (numpy PCG64 seed 500)" % (499 + a.runs),
+            "validation_seeds": [8001 + i for i in range(a.val_seeds)]}
     if a.save:
         save_net(a.save, net, gain, pd_c, meta)
         progress("saved %s" % a.save)
@@ -268,6 +282,13 @@ def main(argv=None):
         "gain": gain, "coarse_prediss_closure": pd_c, "coarse_prediss_plain_tuned": pd_p, "validation": val,
         "eval_steps": a.eval_steps, "eval": ev,
     })
+    summarise(out, ev)
+    net.close()
+    print(json.dumps(out))
+    return out
+
+
+def summarise(out, ev):
     tf = [e["fine_s"] for e in ev]
     tc = [e["closure"]["s"] for e in ev]
     out["speedup_per_time"] = float(np.mean(tf) / np.mean(tc))
@@ -278,6 +299,28 @@ def main(argv=None):
                                              for e in ev))
     out["closure_better_enstrophy"] = int(sum(e["closure"]["enstrophy_error"] < e["no_closure_tuned"]["enstrophy_error"]
                                               for e in ev))
+
+
+def evaluate_saved(a):
+    """Held-out evaluation and step timing of a saved net, with the gain and coarse
+    prediss stored beside its weights (chosen on validation seeds when it was trained)."""
+    net, gain, pd_c, meta = load_net(a.load, a.coarse)
+    pd_p = meta.get("coarse_prediss_plain_tuned", pd_c)
+    fwd_ms, train_ms = net.timing(20)
+    npts = a.coarse * a.coarse * (a.coarse + 1)
+    gflop = CL.GFLOP_FWD_PER_POINT * npts
+    cfg = [("no_closure", 30.0, None), ("no_closure_tuned", pd_p, None), ("closure", pd_c, gain)]
+    ev = evaluate(a.fine, a.coarse, net, [9001 + i for i in range(a.eval_seeds)], a.eval_steps, cfg)
+    st = step_timing(a.fine, a.coarse, net)
+    out = dict(meta)
+    out.update({
+        "loaded": a.load, "forward_ms": fwd_ms, "forward_gflop": gflop, "forward_tflops": gflop / fwd_ms,
+        "forward_mfma_frac": gflop / fwd_ms * 1e12 / 2.5e15, "train_step_ms": train_ms,
+        "step_ms": st, "speedup_per_step": st["fine"] / st["coarse_closure"],
+        "gain": gain, "coarse_prediss_closure": pd_c, "coarse_prediss_plain_tuned": pd_p,
+        "eval_steps": a.eval_steps, "eval": ev,
+    })
+    summarise(out, ev)
     net.close()
     print(json.dumps(out))
     return out

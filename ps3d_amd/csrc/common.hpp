@@ -19,7 +19,7 @@ namespace ps3d {
 constexpr int kMaxPass = 16;
 constexpr int kZAlign = 16;
 
-// Mixed-radix plan: n = prod R[s], radices from {8, 4, 2, 3}.
+// Mixed-radix plan: n = prod R[s], radices from {8, 4, 2, 3, 5}.
 struct FftPlan {
     int n;
     int npass;
@@ -27,7 +27,8 @@ struct FftPlan {
 };
 
 #ifndef PS3D_DEVICE_ONLY
-// Returns false when n has a prime factor other than 2 and 3.
+// Returns false when n has a prime factor other than 2, 3 and 5 (the primes of
+// stafft's factorisen, src/fft/stafft.f90:128-187).
 inline bool make_plan(int n, FftPlan &P) {
     P.n = n;
     P.npass = 0;
@@ -37,6 +38,7 @@ inline bool make_plan(int n, FftPlan &P) {
     while (rem % 4 == 0 && P.npass < kMaxPass) { P.R[P.npass++] = 4; rem /= 4; }
     while (rem % 2 == 0 && P.npass < kMaxPass) { P.R[P.npass++] = 2; rem /= 2; }
     while (rem % 3 == 0 && P.npass < kMaxPass) { P.R[P.npass++] = 3; rem /= 3; }
+    while (rem % 5 == 0 && P.npass < kMaxPass) { P.R[P.npass++] = 5; rem /= 5; }
     return rem == 1;
 }
 #endif
@@ -55,7 +57,7 @@ __host__ __device__ inline double2 cmuli(double2 a, int dir) {
     return dir < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
 }
 
-// In-register DFT of radix R (2, 3, 4, 8): u[s] <- sum_t u[t] exp(dir 2 pi i s t / R)
+// In-register DFT of radix R (2, 3, 4, 5, 8): u[s] <- sum_t u[t] exp(dir 2 pi i s t / R)
 __host__ __device__ inline void dft4(double2 &a0, double2 &a1, double2 &a2, double2 &a3, int dir) {
     double2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = cmuli(csub(a1, a3), dir);
     a0 = cadd(t0, t2);
@@ -87,6 +89,25 @@ __host__ __device__ inline void butterfly(int R, double2 *u, int dir) {
         case 4:
             dft4(u[0], u[1], u[2], u[3], dir);
             break;
+        case 5: {
+            // pairs t, 5-t share cos(2 pi s t/5); their differences carry the sines
+            const double c1 = 0.30901699437494742410229341718281906;    // cos(2 pi/5)
+            const double c2 = -0.80901699437494742410229341718281906;   // cos(4 pi/5)
+            const double s1 = 0.95105651629515357211643933337938214;    // sin(2 pi/5)
+            const double s2 = 0.58778525229247312916870595463907277;    // sin(4 pi/5)
+            const double2 t1 = cadd(u[1], u[4]), t2 = cadd(u[2], u[3]);
+            const double2 t3 = csub(u[1], u[4]), t4 = csub(u[2], u[3]);
+            const double2 a1 = make_double2(u[0].x + c1 * t1.x + c2 * t2.x, u[0].y + c1 * t1.y + c2 * t2.y);
+            const double2 a2 = make_double2(u[0].x + c2 * t1.x + c1 * t2.x, u[0].y + c2 * t1.y + c1 * t2.y);
+            const double2 b1 = cmuli(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), dir);
+            const double2 b2 = cmuli(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), dir);
+            u[0] = cadd(u[0], cadd(t1, t2));
+            u[1] = cadd(a1, b1);
+            u[4] = csub(a1, b1);
+            u[2] = cadd(a2, b2);
+            u[3] = csub(a2, b2);
+            break;
+        }
         case 8: {
             double2 e0 = u[0], e1 = u[2], e2 = u[4], e3 = u[6];
             double2 o0 = u[1], o1 = u[3], o2 = u[5], o3 = u[7];
@@ -145,6 +166,7 @@ __host__ __device__ inline void fft_pass(const double2 *src, double2 *dst, int n
         case 2: fft_pass_r<2>(src, dst, nlines, ls, n, p, tw, dir, e0, de); break;
         case 3: fft_pass_r<3>(src, dst, nlines, ls, n, p, tw, dir, e0, de); break;
         case 4: fft_pass_r<4>(src, dst, nlines, ls, n, p, tw, dir, e0, de); break;
+        case 5: fft_pass_r<5>(src, dst, nlines, ls, n, p, tw, dir, e0, de); break;
         default: fft_pass_r<8>(src, dst, nlines, ls, n, p, tw, dir, e0, de); break;
     }
 }

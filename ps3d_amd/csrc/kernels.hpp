@@ -206,7 +206,15 @@ struct LineArgs {
     uint16_t *nn_in;   // LM_XMAIN: also write the closure input (bf16 [p][16]) when set
     float nn_scale[6];
     int rthreads;      // register-holding modes: 1024 (standard tile), 512 (half depth) or 256 (quarter)
+    int sched;         // transform-only modes: FFT schedule (LS_R8 | LS_FOLD bits)
 };
+// FFT schedules of the transform-only line passes: radix-8-first plan instead of
+// radix-16-first; Hermitian pair of the inverse formed in the first radix pass.
+enum { LS_R8 = 1, LS_FOLD = 2 };
+// the schedule a line length takes unless told otherwise (fold from n = 128 up)
+int line_sched_default(int n);
+// whether a schedule is built for a line length (others fall back to the default)
+bool line_sched_ok(int n, int sched);
 
 size_t line_lds_bytes(int n, int ZC);
 // whether the T-thread register build (T = 256, 512, 1024) exists and holds a tile
@@ -273,6 +281,39 @@ hipError_t launch_stats(const StatsArgs &a, const StatsFin &f, hipStream_t strea
 // device address of the library's own partials array (kStatsMaxBlocks * ST_NP
 // doubles) on device dev
 hipError_t stats_scratch(int dev, double **ptr);
+
+// Domain means of the kinetic energy, helicity and cross-helicity (enum ps3d_moment).
+enum { MO_S_KE = 0, MO_S_HEL, MO_S_XH, MO_MAX_XH2, MO_NP };
+struct MomArgs {
+    int nx, ny, nz, zs;
+    const double *vel[3], *vor[3];   // vor[0] == nullptr: kinetic energy only
+    double *part;                    // nb * MO_NP partials
+    int nb;
+    double ncelli;
+    double *out;                     // PS3D_NMOMENT values
+};
+hipError_t launch_moments(const MomArgs &a, hipStream_t stream);
+
+// Shell binning of genspec's spectrum: squared coefficients of one z-transformed
+// spectral component summed per shell m = int(nint(|K|) / dk), per block.
+constexpr int kSpecWaves = 4;
+struct SpecArgs {
+    const double *f;        // column layout, (x, y)-spectral and z-transformed
+    int nx, ny, nz, zs;
+    const double *rkx, *rky;
+    double rkz1;            // pi / extent(3): rkz(kz) = rkz1 * kz
+    double dki;             // 1 / dk
+    int nbins;              // shells held per block (largest m + 1)
+    double *part;           // nb * nbins sums
+    int *cpart;             // nb * nbins point counts, or nullptr
+    int nb;
+    int accumulate;         // add to part instead of overwriting it
+};
+size_t spec_lds_bytes(int nbins);
+hipError_t launch_spectrum(const SpecArgs &a, hipStream_t stream);
+// per-shell totals over the blocks, in block order
+hipError_t launch_spectrum_fin(const double *part, const int *cpart, int nb, int nbins, double *spec,
+                               long long *cnt, hipStream_t stream);
 
 // closure hooks (closure.hip)
 bool cnn_matches(const ps3d_cnn *c, int nx, int ny, int nz);

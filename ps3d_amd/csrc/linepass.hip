@@ -40,12 +40,13 @@ __host__ __device__ constexpr int zc_reg(int n, int T) {
 __host__ __device__ constexpr bool reg_fits(int n, int T) { return n * zc_reg(n, T) / 2 <= kEpt * T; }
 
 // Register-holding modes run 1024 threads per tile and use the radix-8 plans to
-// stay within 128 VGPRs; transform-only modes use the radix-16 plans.
-template <int N, bool REG = false>
+// stay within 128 VGPRs; transform-only modes use the radix-16 plans unless their
+// schedule asks for the radix-8 ones (R8).
+template <int N, bool REG = false, bool R8 = false>
 __device__ inline double2 *lfft(const Tile &t, const LineArgs &a, double2 *A, double2 *B, int dir) {
     if constexpr (N == 0)
         return fft_lds(A, B, t.nl, t.LS, a.P, a.tw, dir);
-    else if constexpr (REG && CPlan8<N>::ok)
+    else if constexpr ((REG || R8) && CPlan8<N>::ok)
         return cfft_lds<N, CPlan8<N>, false, true>(A, B, t.nl, t.LS, t.twl, dir);
     else
         return cfft_lds<N, CPlan<N>, false, true>(A, B, t.nl, t.LS, t.twl, dir);
@@ -140,16 +141,16 @@ __device__ inline void tile_store(const Tile &t, const LineArgs &a, double *__re
 }
 
 // Hermitian pair -> complex spectrum, then inverse FFT.  Returns result buffer,
-// values to be scaled by 1/n.  With a compile-time plan of length >= 128 the pair
-// is formed inside the first FFT pass.
-template <int N, bool REG = false>
+// values to be scaled by 1/n.  FOLD (a compile-time plan only): the pair is formed
+// inside the first FFT pass instead of in a pass of its own.
+template <int N, bool REG = false, bool FOLD = (N >= 128), bool R8 = false>
 __device__ inline double2 *tile_inverse(const Tile &t, const LineArgs &a, double2 *A, double2 *B) {
     const int n = t.n, h = n / 2;
     const double sn = sqrt((double)n);
-    // (measured: the fold pays from n = 128 up; the small grids of an ensemble run
-    // faster with the separate pass)
-    if constexpr (N >= 128) {
-        if constexpr (REG && CPlan8<N>::ok)
+    // (measured on single grids: the fold pays from n = 128 up, which is the default;
+    // the transform-only passes take it as part of their schedule)
+    if constexpr (FOLD && N > 0) {
+        if constexpr ((REG || R8) && CPlan8<N>::ok)
             return cfft_lds<N, CPlan8<N>, true, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
         else
             return cfft_lds<N, CPlan<N>, true, true>(A, B, t.nl, t.LS, t.twl, +1, sn);
@@ -169,13 +170,13 @@ __device__ inline double2 *tile_inverse(const Tile &t, const LineArgs &a, double
         }
     }
     __syncthreads();
-    return lfft<N, REG>(t, a, A, B, +1);
+    return lfft<N, REG, R8>(t, a, A, B, +1);
 }
 
 // Forward FFT then split into the Hermitian pair, in the returned buffer, scaled.
-template <int N, bool REG = false>
+template <int N, bool REG = false, bool R8 = false>
 __device__ inline double2 *tile_forward(const Tile &t, const LineArgs &a, double2 *A, double2 *B) {
-    double2 *R = lfft<N, REG>(t, a, A, B, -1);
+    double2 *R = lfft<N, REG, R8>(t, a, A, B, -1);
     const int n = t.n, h = n / 2;
     const double sc = 1.0 / sqrt((double)n);
     const int tot = t.nl * (h + 1);
@@ -490,9 +491,12 @@ __device__ inline double eig_absmax_est(double s11, double s12, double s13, doub
 }
 
 // Transform-only modes (forward, inverse, derivative) of nf fields; the global
-// loads of field f+1 are issued before the FFT of field f.
-template <int N>
+// loads of field f+1 are issued before the FFT of field f.  SCHED: the FFT
+// schedule, bit LS_R8 = radix-8-first plan instead of radix-16-first, bit LS_FOLD
+// = Hermitian pair of the inverse formed in the first radix pass.
+template <int N, int SCHED>
 __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
+    constexpr bool R8 = (SCHED & LS_R8) != 0, FOLD = (SCHED & LS_FOLD) != 0;
     extern __shared__ double2 lsmem[];
     if (gated_off(a.g.gate)) return;
     Tile t = make_tile(a);
@@ -509,10 +513,10 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
             tile_load(t, a, a.in[f], A);
             __syncthreads();
             if (a.mode == LM_FWD) {
-                double2 *R = tile_forward<N>(t, a, A, B);
+                double2 *R = tile_forward<N, false, R8>(t, a, A, B);
                 tile_store(t, a, a.out[f], R, 1.0);
             } else if (a.mode == LM_INV) {
-                double2 *R = tile_inverse<N>(t, a, A, B);
+                double2 *R = tile_inverse<N, false, FOLD, R8>(t, a, A, B);
                 tile_store(t, a, a.out[f], R, 1.0 / (double)t.n);
             } else {
                 tile_diff(t, a, A, B);
@@ -558,10 +562,10 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
             __syncthreads();
             if (f + 1 < a.nf) gload(a.in[f + 1]);
             if (a.mode == LM_FWD) {
-                double2 *R = lfft<N>(t, a, A, B, -1);
+                double2 *R = lfft<N, false, R8>(t, a, A, B, -1);
                 tile_store_fwd<N>(t, a, a.out[f], R);
             } else {
-                double2 *R = tile_inverse<N>(t, a, A, B);
+                double2 *R = tile_inverse<N, false, FOLD, R8>(t, a, A, B);
                 tile_store(t, a, a.out[f], R, 1.0 / (double)t.n);
             }
             __syncthreads();
@@ -976,17 +980,19 @@ hipError_t configure_linepass() {
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_INV_Y1, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XMAIN, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XADAPT, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-#define CFG(N_)                                                                                              \
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_simple<N_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds); \
-    CFGT(N_, kLineThreads)
+#define CFGS(N_, S_)                                                                                         \
+    if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_simple<N_, S_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+#define CFG(N_) CFGS(N_, 0) CFGS(N_, 1) CFGS(N_, 2) CFGS(N_, 3) CFGT(N_, kLineThreads)
 #define CFG2(N_) CFG(N_) CFGT(N_, 512)
 #define CFG4(N_) CFGT(N_, 256)
-    CFG(0)
+    CFGS(0, 0)
+    CFGT(0, kLineThreads)
     PS3D_LINE_SIZES(CFG2)
     PS3D_LINE_SIZES_Q(CFG4)
 #undef CFG4
 #undef CFG2
 #undef CFG
+#undef CFGS
 #undef CFGT
     return e;
 }
@@ -1008,8 +1014,18 @@ static void launch_line_t(const LineArgs &a, hipStream_t stream) {
     size_t lds = line_lds_bytes(a.n, a.ZC);
     if ((a.mode == LM_XMAIN || a.mode == LM_XADAPT) && N > 0)
         lds += (size_t)2 * (a.ZC / 2) * line_ls(a.n) * sizeof(double2);   // field pairs
-    if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF)
-        hipLaunchKernelGGL(k_line_simple<N>, dim3(blocks, nb), dim3(256), lds, stream, a);
+    if (a.mode == LM_FWD || a.mode == LM_INV || a.mode == LM_DIFF) {
+        if constexpr (N == 0) {
+            hipLaunchKernelGGL((k_line_simple<0, 0>), dim3(blocks, nb), dim3(256), lds, stream, a);
+        } else {
+            switch (a.sched & 3) {
+                case 0: hipLaunchKernelGGL((k_line_simple<N, 0>), dim3(blocks, nb), dim3(256), lds, stream, a); break;
+                case 1: hipLaunchKernelGGL((k_line_simple<N, 1>), dim3(blocks, nb), dim3(256), lds, stream, a); break;
+                case 2: hipLaunchKernelGGL((k_line_simple<N, 2>), dim3(blocks, nb), dim3(256), lds, stream, a); break;
+                default: hipLaunchKernelGGL((k_line_simple<N, 3>), dim3(blocks, nb), dim3(256), lds, stream, a); break;
+            }
+        }
+    }
     else if (a.rthreads == 512 && N > 0)
         launch_reg_t<N, 512>(a, stream, blocks, nb, lds);
     else if constexpr (N > 0 && N <= 512) {
@@ -1034,6 +1050,21 @@ bool line_reg_ok(int n, int T) {
 }
 
 int line_reg_zc(int n, int T) { return zc_reg(n, T); }
+
+int line_sched_default(int n) { return n >= 128 ? LS_FOLD : 0; }
+
+bool line_sched_ok(int n, int sched) {
+    if (sched < 0 || sched > 3) return false;
+    bool built = false, r8 = false;
+    switch (n) {
+#define CASE(N_) case N_: built = true; r8 = CPlan8<N_>::ok; break;
+        PS3D_LINE_SIZES(CASE)
+#undef CASE
+        default: break;
+    }
+    if (!built) return sched == 0;
+    return !(sched & LS_R8) || r8;
+}
 
 hipError_t launch_line(const LineArgs &a, int nthreads, hipStream_t stream) {
     (void)nthreads;

@@ -81,6 +81,7 @@ struct ps3d_handle {
     // tile, or 512 on half of it (two blocks per CU)
     LineCfg lxm{}, lxr{}, lyr{};
     int rtm = 1024, rtx = 1024, rty = 1024;
+    int lsx = 0, lsy = 0;          // FFT schedules of the transform-only x and y line passes
     double visc = 0.0;
     bool have_state = false, have_phys = false, have_adapt = false;
     // batched instances of this grid (ensemble group): instance b's data sit at
@@ -94,6 +95,12 @@ struct ps3d_handle {
     bool have_derived = false;
     bool derived_fresh = false;   // pres / delta belong to the state of the last vor2vel
     double *stats_part = nullptr, *stats_out = nullptr;   // ps3d_gpu_field_stats, allocated on first use
+    // ps3d_gpu_spectrum / ps3d_gpu_moments buffers, allocated on first use: per-block
+    // shell sums and counts (spec_cap shells), then the totals and the moments
+    double *spec_part = nullptr, *spec_out = nullptr;
+    int *spec_cpart = nullptr;
+    long long *spec_cnt = nullptr;
+    int spec_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_steps = 0;
@@ -163,10 +170,12 @@ static std::vector<double2> twiddles(int n) {
     return t;
 }
 
-static bool factor23(int n) {
+// stafft's factorisen (src/fft/stafft.f90:128-187) takes the primes 2, 3 and 5.
+static bool factor235(int n) {
     if (n < 2) return false;
     while (n % 2 == 0) n /= 2;
     while (n % 3 == 0) n /= 3;
+    while (n % 5 == 0) n /= 5;
     return n == 1;
 }
 
@@ -227,6 +236,10 @@ static void free_handle(ps3d_handle *h) {
     hipFree(h->delta);
     hipFree(h->stats_part);
     hipFree(h->stats_out);
+    hipFree(h->spec_part);
+    hipFree(h->spec_out);
+    hipFree(h->spec_cpart);
+    hipFree(h->spec_cnt);
     hipFree(h->scal);
     hipFree(h->party);
     hipFree(h->partx);
@@ -259,8 +272,8 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
     if (strict) {
         if (nx < 2 || ny < 2 || nz < 2 || (nx & 1) || (ny & 1) || (nz & 1))
             return fail(PS3D_E_ARG, "nx, ny, nz must be even and >= 2");
-        if (!factor23(nx) || !factor23(ny) || !factor23(nz))
-            return fail(PS3D_E_FACTOR, "Factorisation not possible: grid lengths must be 2^a 3^b");
+        if (!factor235(nx) || !factor235(ny) || !factor235(nz))
+            return fail(PS3D_E_FACTOR, "Factorisation not possible: grid lengths must be 2^a 3^b 5^c");
         if (nx > 4096 || ny > 4096 || nz > 1024) return fail(PS3D_E_UNSUPPORTED, "grid too large (nx,ny <= 4096, nz <= 1024)");
         for (int d = 0; d < 3; ++d)
             if (!(p->extent[d] > 0.0)) return fail(PS3D_E_ARG, "Domain length equal to zero not allowed");
@@ -316,7 +329,7 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
         (st = upload(&h->fily, fy)) || (st = upload(&h->filz, fz)))
         return st;
     if (!make_plan(nx, h->px) || !make_plan(ny, h->py) || !make_plan(2 * nz, h->pz))
-        return fail(PS3D_E_FACTOR, "Factorisation not possible: grid lengths must be 2^a 3^b");
+        return fail(PS3D_E_FACTOR, "Factorisation not possible: grid lengths must be 2^a 3^b 5^c");
     if ((st = upload(&h->twx, twiddles(nx))) || (st = upload(&h->twy, twiddles(ny))) ||
         (st = upload(&h->twz, twiddles(2 * nz))))
         return st;
@@ -350,8 +363,13 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
         // register variant: the two-waves-per-SIMD builds (no spills, the update's
         // operands loaded ahead) measure fastest at nz = 256 and 512
         int var = 2;
-        if (ev && ev[0] >= '1' && ev[0] <= '3') var = ev[0] - '0';
-        h->zl.wave = (ps3d::zwave_supported(nz) && !off) ? var : 0;
+        const bool asked = ev && ev[0] >= '1' && ev[0] <= '3';
+        if (asked) var = ev[0] - '0';
+        // nz = 64: the quad kernels measure faster on a single grid (0.61 against
+        // 0.99 ms per 64^3 step); the wave kernels there serve when asked for and as
+        // a candidate of the ensemble tuner
+        const bool dflt = nz >= 128;
+        h->zl.wave = (ps3d::zwave_supported(nz) && !off && (dflt || asked)) ? var : 0;
         const char *fz = std::getenv("PS3D_ZFUSE");
         h->zfuse = !(fz && fz[0] == '0');
     }
@@ -369,6 +387,11 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
         if (std::max(nx, ny) <= 96) tm = ty = tx = 256;
         if (const char *ev = std::getenv("PS3D_LINE_T")) std::sscanf(ev, "%d,%d,%d", &tm, &ty, &tx);
         apply_line_choice(h, tm, ty, tx);
+        // FFT schedules of the transform-only passes; PS3D_LINE_SCHED="x,y" overrides
+        int sx = line_sched_default(nx), sy = line_sched_default(ny);
+        if (const char *ev = std::getenv("PS3D_LINE_SCHED")) std::sscanf(ev, "%d,%d", &sx, &sy);
+        h->lsx = line_sched_ok(nx, sx) ? sx : line_sched_default(nx);
+        h->lsy = line_sched_ok(ny, sy) ? sy : line_sched_default(ny);
     }
 
     HIPCHK(ps3d::configure_zpass());
@@ -448,6 +471,7 @@ static LineArgs line_args(const ps3d_handle *h, int axis, int mode) {
     const LineCfg &c = mode == LM_XMAIN ? h->lxm : mode == LM_XADAPT ? h->lxr : mode == LM_INV_Y1 ? h->lyr
                      : (axis == 0 ? h->lx : h->ly);
     a.rthreads = mode == LM_XMAIN ? h->rtm : mode == LM_XADAPT ? h->rtx : mode == LM_INV_Y1 ? h->rty : 256;
+    a.sched = axis == 0 ? h->lsx : h->lsy;
     a.g = h->g;
     a.axis = axis;
     a.n = c.n;
@@ -1348,6 +1372,169 @@ int ps3d_gpu_field_stats_noalloc(ps3d_handle *h, const double *diag, double *sta
     return field_stats_launch(h, diag, scr, stats_dev);
 }
 
+// ---------------------------------------------------------------------------
+// genspec's spectrum and nc_reader's derived means
+// ---------------------------------------------------------------------------
+constexpr int kSpecMaxBlocks = 1024;
+
+static int spec_buffers(ps3d_handle *h, int nbins) {
+    if (nbins <= h->spec_cap && h->spec_part) return 0;
+    hipFree(h->spec_part);
+    hipFree(h->spec_out);
+    hipFree(h->spec_cpart);
+    hipFree(h->spec_cnt);
+    h->spec_part = h->spec_out = nullptr;
+    h->spec_cpart = nullptr;
+    h->spec_cnt = nullptr;
+    h->spec_cap = 0;
+    const size_t np = (size_t)kSpecMaxBlocks * nbins;
+    if (hipMalloc((void **)&h->spec_part, sizeof(double) * std::max<size_t>(np, (size_t)kStatsMaxBlocks * MO_NP)) != hipSuccess ||
+        hipMalloc((void **)&h->spec_cpart, sizeof(int) * np) != hipSuccess ||
+        hipMalloc((void **)&h->spec_out, sizeof(double) * (nbins + PS3D_NMOMENT)) != hipSuccess ||
+        hipMalloc((void **)&h->spec_cnt, sizeof(long long) * nbins) != hipSuccess)
+        return fail(PS3D_E_NOMEM, "spectrum buffers");
+    h->spec_cap = nbins;
+    return 0;
+}
+
+// kinetic energy, helicity and cross-helicity of the handle's physical fields into
+// out (device, PS3D_NMOMENT doubles); with_vor = false: kinetic energy only
+static int moments_launch(ps3d_handle *h, bool with_vor, double *part, double *out) {
+    MomArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.nx = h->g.nx;
+    a.ny = h->g.ny;
+    a.nz = h->g.nz;
+    a.zs = h->g.zs;
+    for (int c = 0; c < 3; ++c) {
+        a.vel[c] = h->vel[c];
+        a.vor[c] = with_vor ? h->vor[c] : nullptr;
+    }
+    a.part = part;
+    a.nb = stats_blocks(h->g.nx, h->g.ny);
+    a.ncelli = 1.0 / ((double)h->g.nx * h->g.ny * h->g.nz);
+    a.out = out;
+    HIPCHK(launch_moments(a, h->stream));
+    return 0;
+}
+
+int ps3d_gpu_moments(ps3d_handle *h, double *out) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
+    if (!h || !out) return fail(PS3D_E_ARG, "null argument");
+    if (!h->have_phys) return fail(PS3D_E_STATE, "moments need a preceding vor2vel");
+    int st;
+    if ((st = spec_buffers(h, std::max(1, h->spec_cap)))) return st;
+    double *res = h->spec_out + h->spec_cap;
+    if ((st = moments_launch(h, true, h->spec_part, res))) return st;
+    HIPCHK(hipMemcpyAsync(out, res, sizeof(double) * PS3D_NMOMENT, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// genspec.f90:62-81: largest shell radius kmax = nint(|K|max) and the shell width dk.
+// The largest |K|^2 is the sum of the three largest squared wavenumbers (each term
+// and the rounded sum are monotone), so no pass over the grid is needed.
+static void spectrum_shells(const ps3d_handle *h, int *kmax, double *dk, double *dki, int *nbins) {
+#pragma clang fp contract(off)
+    double mx = 0.0, my = 0.0;
+    for (double v : h->h_rkx) mx = std::max(mx, v * v);
+    for (double v : h->h_rky) my = std::max(my, v * v);
+    const double rz = (3.141592653589793238462643383279502884 / h->p.extent[2]) * (double)h->g.nz;
+    const double k2 = mx + my + rz * rz;   // rounded as the kernel rounds (no contraction)
+    *kmax = (int)std::lround(std::sqrt((double)k2));
+    const double nx = h->g.nx, ny = h->g.ny, nz = h->g.nz;
+    *dk = (double)*kmax / std::sqrt((0.5 * nx) * (0.5 * nx) + (0.5 * ny) * (0.5 * ny) + nz * nz);
+    *dki = 1.0 / *dk;
+    // shells the binning can reach: m <= int(kmax * dki)
+    *nbins = std::max(*kmax, (int)((double)*kmax * *dki)) + 1;
+}
+
+// genspec.f90:101-117: shell m scaled by the volume of its spherical shell over its
+// point count, then the whole spectrum by ke / sum(spec dk).
+static void spectrum_scale(int kmax, double dk, double ke, const double *raw, const long long *cnt, double *spec) {
+#pragma clang fp contract(off)
+    const double prefactor = 4.0 / 3.0 * 3.141592653589793238462643383279502884 * (dk * dk * dk);
+    double tot = 0.0;
+    for (int m = 0; m <= kmax; ++m) {
+        const long long m1 = m + 1, m0 = m;
+        spec[m] = cnt[m] > 0 ? raw[m] * prefactor * (double)(m1 * m1 * m1 - m0 * m0 * m0) / (double)cnt[m] : 0.0;
+        tot += spec[m] * dk;
+    }
+    const double snorm = ke / tot;
+    for (int m = 0; m <= kmax; ++m) spec[m] *= snorm;
+}
+
+int ps3d_gpu_spectrum(ps3d_handle *h, const double *vel, int cap, int *nshell, double *dk, double *spec,
+                      long long *num) {
+    if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
+    if (!h || !nshell) return fail(PS3D_E_ARG, "null argument");
+    int kmax, nbins;
+    double dkv, dki;
+    spectrum_shells(h, &kmax, &dkv, &dki, &nbins);
+    *nshell = kmax + 1;
+    if (dk) *dk = dkv;
+    if (!spec) return 0;                    // size query
+    if (cap < kmax + 1) return fail(PS3D_E_ARG, "spectrum: cap is smaller than the number of shells");
+    if (!(dkv > 0.0)) return fail(PS3D_E_ARG, "spectrum: no nonzero wavenumber");
+    if (spec_lds_bytes(nbins) > 160 * 1024) return fail(PS3D_E_UNSUPPORTED, "spectrum: too many shells for the LDS histogram");
+    int st;
+    if (vel) {
+        // a velocity from the caller replaces the handle's physical velocity
+        drop_step_graph(h);
+        const size_t hv = (size_t)h->g.nx * h->g.ny * (h->g.nz + 1);
+        for (int c = 0; c < 3; ++c)
+            if ((st = copy_in(h, h->vel[c], vel + c * hv))) return st;
+        h->have_phys = h->have_adapt = h->derived_fresh = false;
+    } else if (!h->have_phys) {
+        return fail(PS3D_E_STATE, "spectrum needs a preceding vor2vel (or a velocity)");
+    }
+    if ((st = spec_buffers(h, nbins))) return st;
+    h->have_grad = false;                   // the y-derivative volumes serve as scratch
+    double *S = h->grad[0], *T = h->grad[1];
+    double *s1[1] = {S};
+    double *res = h->spec_out + h->spec_cap;
+    // ke = get_kinetic_energy() of the same velocity (genspec.f90:46)
+    if ((st = moments_launch(h, false, h->spec_part, res))) return st;
+    HIPCHK(hipMemcpyAsync(spec, res, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const double ke = spec[0];
+    SpecArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.nx = h->g.nx;
+    a.ny = h->g.ny;
+    a.nz = h->g.nz;
+    a.zs = h->g.zs;
+    a.rkx = h->rkx;
+    a.rky = h->rky;
+    a.rkz1 = 3.141592653589793238462643383279502884 / h->p.extent[2];
+    a.dki = dki;
+    a.nbins = nbins;
+    a.part = h->spec_part;
+    a.nb = (int)std::min<long>(kSpecMaxBlocks, std::max<long>(1, (long)h->g.nx * h->g.ny / 16));
+    a.f = T;
+    // (1) fftxyp2s of each component, dct of u and v, dst of w (genspec.f90:50-60);
+    // (3) squared amplitudes summed per shell, point counts with the first component
+    for (int c = 0; c < 3; ++c) {
+        double *v1[1] = {h->vel[c]};
+        if ((st = line_fields(h, 1, LM_FWD, 1, v1, s1)) || (st = line_fields(h, 0, LM_FWD, 1, s1, s1))) return st;
+        HIPCHK(launch_zop(h->zl, zop_args(h, c < 2 ? ZOP_DCT : ZOP_DST, S, T), h->stream));
+        a.accumulate = c > 0;
+        a.cpart = c == 0 ? h->spec_cpart : nullptr;
+        HIPCHK(launch_spectrum(a, h->stream));
+    }
+    HIPCHK(launch_spectrum_fin(h->spec_part, h->spec_cpart, a.nb, nbins, h->spec_out, h->spec_cnt, h->stream));
+    std::vector<double> raw(kmax + 1);
+    std::vector<long long> cnt(kmax + 1);
+    HIPCHK(hipMemcpyAsync(raw.data(), h->spec_out, sizeof(double) * (kmax + 1), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(cnt.data(), h->spec_cnt, sizeof(long long) * (kmax + 1), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // shells beyond kmax (reachable only when dk < 1) are not part of genspec's spectrum
+    spectrum_scale(kmax, dkv, ke, raw.data(), cnt.data(), spec);
+    if (num)
+        for (int m = 0; m <= kmax; ++m) num[m] = cnt[m];
+    return 0;
+}
+
 int ps3d_gpu_get(ps3d_handle *h, int which, double *host) {
     if (h && h->nbatch != 1) return fail(PS3D_E_STATE, "batched handle: use the _batch entry points");
     if (!h || !host) return fail(PS3D_E_ARG, "null argument");
@@ -1437,7 +1624,7 @@ int ps3d_gpu_op(ps3d_handle *h, int op, const double *in, double *out) {
 
 // Batched stafft transforms on x(m, n) (m fastest) through the line / z kernels.
 int ps3d_gpu_fft1d(int kind, int m, int n, double *x) {
-    if (!x || m < 1 || n < 2 || (n & 1) || !factor23(n)) return fail(PS3D_E_FACTOR, "fft1d: n must be even and 2^a 3^b");
+    if (!x || m < 1 || n < 2 || (n & 1) || !factor235(n)) return fail(PS3D_E_FACTOR, "fft1d: n must be even and 2^a 3^b 5^c");
     if (kind < 0 || kind > 3) return fail(PS3D_E_ARG, "fft1d: unknown kind");
     ps3d_params p;
     ps3d_params_default(&p);
@@ -1862,6 +2049,29 @@ int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice) {
             }
             apply_line_choice(h, cur[0], cur[1], cur[2]);
         }
+        // FFT schedule of the transform-only passes: y lines, then x lines
+        for (int axis = 1; axis >= 0 && !st; --axis) {
+            const int n = axis == 0 ? h->g.nx : h->g.ny;
+            int &cur = axis == 0 ? h->lsx : h->lsy;
+            int bests = cur;
+            float bt = 1e30f;
+            for (int sc = 0; sc < 4; ++sc) {
+                if (!line_sched_ok(n, sc)) continue;
+                cur = sc;
+                if ((st = enqueue_steps(h, 1))) break;
+                HIPCHK(hipEventRecord(h->ev0, h->stream));
+                if ((st = enqueue_steps(h, steps))) break;
+                HIPCHK(hipEventRecord(h->ev1, h->stream));
+                HIPCHK(hipStreamSynchronize(h->stream));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+                if (ms < bt) {
+                    bt = ms;
+                    bests = sc;
+                }
+            }
+            cur = bests;
+        }
         for (size_t b = 0; b < nb; ++b)
             HIPCHK(hipMemcpyAsync(h->svor[0] + b * h->b_vol, bk + b * sv, sizeof(double) * sv, hipMemcpyDeviceToDevice,
                                   h->stream));
@@ -1873,6 +2083,13 @@ int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice) {
         if (choice) choice[gi] = best;
     }
     drop_graph(e);
+    return 0;
+}
+
+int ps3d_gpu_ensemble_fft_schedule(ps3d_ensemble *e, int g, int *sched) {
+    if (!e || !sched || g < 0 || g >= (int)e->g.size()) return fail(PS3D_E_ARG, "invalid argument");
+    sched[0] = e->g[g]->lsx;
+    sched[1] = e->g[g]->lsy;
     return 0;
 }
 

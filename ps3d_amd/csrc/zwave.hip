@@ -14,8 +14,8 @@
 // and only the x/y derivative products exchange data between the four waves
 // (one LDS exchange with two block barriers).
 //
-// Sizes: nz in {128, 256, 512} (P = 4, 8, 16).  Other nz use the quad-per-thread
-// kernels of zpass.hip.
+// Sizes: nz in {64, 128, 256, 512} (P = 2, 4, 8, 16).  Other nz use the
+// quad-per-thread kernels of zpass.hip.
 #include "kernels.hpp"
 #include "lfft.hpp"
 #include "zmath.hpp"
@@ -314,8 +314,15 @@ __device__ inline void wfft(double2 (&x)[P], double2 *__restrict__ ws, const WTw
             x[2 * j] = u[0][j];
             x[2 * j + 1] = u[1][j];
         }
+    } else if constexpr (P == 2) {
+        // item it = lane mod 16 holds X[it + 16 k2b] in u[0][k2b]; element lane + 64 j
+        // is X[(lane mod 16) + 16 ((lane >> 4) + 4 j)]: this lane's own item
+        const int h = lane >> 4;
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            x[j] = h == 0 ? u[0][4 * j] : h == 1 ? u[0][4 * j + 1] : h == 2 ? u[0][4 * j + 2] : u[0][4 * j + 3];
     } else {
-        static_assert(P == 4, "wave FFT lengths: 256, 512, 1024");
+        static_assert(P == 4, "wave FFT lengths: 128, 256, 512, 1024");
         // lanes < 32 hold X[lane + 32 k2b]; element lane + 64 j with lane >= 32 is
         // X[(lane - 32) + 32 (2 j + 1)]
         const int src = lane & 31;
@@ -962,11 +969,11 @@ __global__ void __launch_bounds__(256) k_zop_w(ZOpArgs a) {
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-bool zwave_supported(int nz) { return nz == 128 || nz == 256 || nz == 512; }
+bool zwave_supported(int nz) { return nz == 64 || nz == 128 || nz == 256 || nz == 512; }
 
 size_t zwave_lds_bytes(int nz) { return 4 * (size_t)72 * (size_t)(nz / 32) * sizeof(double2); }
 
-#define PS3D_ZW_SIZES(X) X(128) X(256) X(512)
+#define PS3D_ZW_SIZES(X) X(64) X(128) X(256) X(512)
 
 hipError_t configure_zwave() {
     hipError_t e = hipSuccess;
@@ -999,6 +1006,7 @@ hipError_t configure_zwave() {
 
 #define ZW_DISPATCH(KERNEL, NQ, NB, ARGS)                                                                   \
     switch (nz) {                                                                                       \
+        case 64: hipLaunchKernelGGL(KERNEL<64>, dim3(NQ, NB), dim3(256), zwave_lds_bytes(64), stream, ARGS); break; \
         case 128: hipLaunchKernelGGL(KERNEL<128>, dim3(NQ, NB), dim3(256), zwave_lds_bytes(128), stream, ARGS); break; \
         case 256: hipLaunchKernelGGL(KERNEL<256>, dim3(NQ, NB), dim3(256), zwave_lds_bytes(256), stream, ARGS); break; \
         case 512: hipLaunchKernelGGL(KERNEL<512>, dim3(NQ, NB), dim3(256), zwave_lds_bytes(512), stream, ARGS); break; \
@@ -1009,9 +1017,11 @@ hipError_t configure_zwave() {
 // at nz = 256); 2: both free; 3: both at three waves per SIMD
 #define ZW_DISPATCH2(KERNEL, NQ, NB, ARGS, OCC3)                                                        \
     switch (nz * 10 + ((OCC3) ? 3 : 1)) {                                                          \
+        case 641: hipLaunchKernelGGL((KERNEL<64, 1>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(64), stream, ARGS); break; \
         case 1281: hipLaunchKernelGGL((KERNEL<128, 1>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(128), stream, ARGS); break; \
         case 2561: hipLaunchKernelGGL((KERNEL<256, 1>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(256), stream, ARGS); break; \
         case 5121: hipLaunchKernelGGL((KERNEL<512, 1>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(512), stream, ARGS); break; \
+        case 643: hipLaunchKernelGGL((KERNEL<64, 3>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(64), stream, ARGS); break; \
         case 1283: hipLaunchKernelGGL((KERNEL<128, 3>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(128), stream, ARGS); break; \
         case 2563: hipLaunchKernelGGL((KERNEL<256, 3>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(256), stream, ARGS); break; \
         case 5123: hipLaunchKernelGGL((KERNEL<512, 3>), dim3(NQ, NB), dim3(256), zwave_lds_bytes(512), stream, ARGS); break; \
@@ -1035,7 +1045,7 @@ hipError_t launch_ztend_zinv_w(int nz, int variant, int nquads, const ZTendArgs 
     case NZ_ * 10 + O_:                                                                                      \
         hipLaunchKernelGGL((k_ztend_zinv_w<NZ_, O_>), dim3(nquads, nb), dim3(256), zwave_lds_bytes(NZ_), stream, t, z); \
         break;
-        ZTZ(128, 1) ZTZ(256, 1) ZTZ(512, 1) ZTZ(128, 3) ZTZ(256, 3) ZTZ(512, 3)
+        ZTZ(64, 1) ZTZ(128, 1) ZTZ(256, 1) ZTZ(512, 1) ZTZ(64, 3) ZTZ(128, 3) ZTZ(256, 3) ZTZ(512, 3)
 #undef ZTZ
         default: return hipErrorInvalidValue;
     }

@@ -5,8 +5,9 @@
 Reads the record's velocity (read_netcdf_fields, src/fields/field_netcdf.f90:276-433;
 --step counts from 1, the default -1 takes the last record), forms ps3d's spectrum
 with the library's transforms (fftxyp2s, dct of u and v, dst of w; genspec.f90:50-60),
-bins the squared amplitudes in shells of K = sqrt(kx^2 + ky^2 + kz^2), normalises the
-spectrum to the domain-averaged kinetic energy (genspec.f90:62-124) and writes
+bins the squared amplitudes in shells of K = sqrt(kx^2 + ky^2 + kz^2) and normalises the
+spectrum to the domain-averaged kinetic energy (genspec.f90:62-124), all on the GPU
+(ps3d_gpu_spectrum), and writes
 <filename without extension>_spectrum.asc with genspec's header (genspec.f90:140-165).
 An existing spectrum file is not overwritten.
 """
@@ -14,19 +15,9 @@ import argparse
 import os
 import sys
 
-import numpy as np
 
-from . import closure, ncio
+from . import ncio
 from . import gpu as G
-
-
-def kinetic_energy(vel):
-    """get_kinetic_energy (src/fields/field_diagnostics.f90:85-122): half the mean of
-    |u|^2 with the lid levels weighted 1/2, over nx ny nz cells."""
-    e = np.sum(vel ** 2, axis=0)                     # (nx, ny, nz+1)
-    nx, ny, nz1 = e.shape
-    tot = 0.5 * e[:, :, 1:-1].sum() + 0.25 * (e[:, :, 0].sum() + e[:, :, -1].sum())
-    return tot / (nx * ny * (nz1 - 1))
 
 
 def spectrum_of_file(filename, step=-1):
@@ -36,7 +27,8 @@ def spectrum_of_file(filename, step=-1):
     _, _, vel = ncio.read_fields(filename, nx, ny, nz, step=step, velocity=True)
     p = G.make_params(nx, ny, nz, lower=origin, extent=extent)
     with G.Ps3dGpu(p) as solver:
-        return closure.velocity_spectrum(solver, vel, kinetic_energy(vel))
+        k, spec, _ = solver.spectrum(vel)
+        return k, spec
 
 
 def write_spectrum(path, k, spec):

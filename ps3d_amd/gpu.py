@@ -35,6 +35,8 @@ STAT_NAMES = ["ke", "en", "vortmax", "vortrms", "vorch", "x_vormean", "y_vormean
               "uoxmax", "loxmax", "uoymax", "loymax", "uozmax", "lozmax", "usuhmax", "usgmax", "lsgmax",
               "uszrms", "usdeltarms", "rolling_mean_gmax", "rolling_mean_bfmax", "ri_min", "ro_min", "ro_max"]
 NSTAT = len(STAT_NAMES)
+# enum ps3d_moment
+MOMENT_NAMES = ["ke", "helicity", "cross_helicity", "cross_helicity_max"]
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
@@ -77,6 +79,9 @@ def lib():
         L.ps3d_gpu_resample.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.ps3d_gpu_viscosity.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.ps3d_gpu_field_stats.argtypes = [C.c_void_p, C.c_void_p, _dp]
+        L.ps3d_gpu_moments.argtypes = [C.c_void_p, _dp]
+        L.ps3d_gpu_spectrum.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_double), C.c_void_p, C.c_void_p]
         L.ps3d_gpu_create_batch.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.ps3d_gpu_batch_size.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.ps3d_gpu_set_vorticity_batch.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -88,6 +93,7 @@ def lib():
         L.ps3d_gpu_ensemble_tune.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.ps3d_gpu_ensemble_set_choice.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.ps3d_gpu_ensemble_line_choice.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.ps3d_gpu_ensemble_fft_schedule.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.ps3d_gpu_ensemble_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
@@ -177,6 +183,30 @@ class Ps3dGpu:
         d = None if diag is None else np.ascontiguousarray(diag, dtype=np.float64)
         _check(lib().ps3d_gpu_field_stats(self.h, None if d is None else d.ctypes.data, out))
         return out
+
+    def moments(self):
+        """{ke, helicity, cross_helicity, cross_helicity_max} of the state of the
+        last vor2vel: domain means of |u|^2/2, u.w and |u x w|, and the largest |u x w|."""
+        out = np.zeros(len(MOMENT_NAMES))
+        _check(lib().ps3d_gpu_moments(self.h, out))
+        return dict(zip(MOMENT_NAMES, out.tolist()))
+
+    def spectrum(self, vel=None):
+        """genspec's spectrum: (shell wavenumbers m dk, spectrum, point counts), of the
+        velocity of the last vor2vel or of vel (3 volumes), which then replaces the
+        handle's physical velocity."""
+        ns, dk = C.c_int(), C.c_double()
+        _check(lib().ps3d_gpu_spectrum(self.h, None, 0, C.byref(ns), C.byref(dk), None, None))
+        spec = np.zeros(ns.value)
+        num = np.zeros(ns.value, dtype=np.int64)
+        v = None
+        if vel is not None:
+            v = np.ascontiguousarray(vel, dtype=np.float64)
+            if v.shape != (3, self.nx, self.ny, self.nz + 1):
+                raise ValueError("vel must have shape (3, nx, ny, nz + 1)")
+        _check(lib().ps3d_gpu_spectrum(self.h, None if v is None else v.ctypes.data, ns.value, C.byref(ns),
+                                       C.byref(dk), spec.ctypes.data, num.ctypes.data))
+        return np.arange(ns.value) * dk.value, spec, num
 
     def viscosity(self):
         """(visc, prefactor of the last adapt): column (k, l) decays at the rate
@@ -375,6 +405,17 @@ class Ps3dEnsemble:
             t = (C.c_int * 3)()
             _check(lib().ps3d_gpu_ensemble_line_choice(self.e, g, t))
             out.append(tuple(t))
+        return out
+
+    def fft_schedules(self):
+        """(x lines, y lines) FFT schedule of each group's transform-only passes as
+        names: radix-16 or radix-8 first plan, Hermitian pair folded into the first
+        pass or formed in a pass of its own."""
+        out = []
+        for g in range(len(self.groups)):
+            t = (C.c_int * 2)()
+            _check(lib().ps3d_gpu_ensemble_fft_schedule(self.e, g, t))
+            out.append(tuple(("r8" if v & 1 else "r16") + ("+fold" if v & 2 else "+pairpass") for v in t))
         return out
 
     def set_choice(self, g, code):

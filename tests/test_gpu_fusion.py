@@ -36,12 +36,12 @@ def run(G, shape, env, nsteps=5, seed=6):
     return rows, s
 
 
-@pytest.mark.parametrize("shape", [(32, 32, 128), (24, 16, 256), (16, 16, 512)])
+@pytest.mark.parametrize("shape", [(32, 32, 64), (32, 32, 128), (24, 16, 256), (16, 16, 512)])
 def test_fused_tendency_and_inversion_match_separate_passes(shape):
     from ps3d_amd import gpu as G
     print("fusion", shape, flush=True)
-    ra, sa = run(G, shape, {"PS3D_ZFUSE": "1"})
-    rb, sb = run(G, shape, {"PS3D_ZFUSE": "0"})
+    ra, sa = run(G, shape, {"PS3D_ZFUSE": "1", "PS3D_ZWAVE": "2"})
+    rb, sb = run(G, shape, {"PS3D_ZFUSE": "0", "PS3D_ZWAVE": "2"})
     for c in range(3):
         assert np.abs(sa[c] - sb[c]).max() <= 1e-13 * np.abs(sb[c]).max()
     np.testing.assert_allclose(ra, rb, rtol=1e-13, atol=1e-13 * np.abs(rb).max())

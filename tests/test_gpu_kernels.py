@@ -27,7 +27,7 @@ def pair(oracle_mod, G, g, **kw):
     return oracle_mod.Oracle(po), G.Ps3dGpu(pg)
 
 
-@pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128, 256, 512])
+@pytest.mark.parametrize("n", [10, 16, 20, 32, 40, 48, 50, 64, 80, 96, 100, 128, 160, 250, 256, 320, 500, 512])
 @pytest.mark.parametrize("m", [1, 6, 17])
 def test_fft1d_matches_oracle(oracle_mod, G, n, m):
     print("fft1d n=%d m=%d" % (n, m), flush=True)

@@ -1,0 +1,102 @@
+"""genspec's spectrum (ps3d_gpu_spectrum, src/genspec.f90:44-124) and the kinetic
+energy / helicity / cross-helicity means (ps3d_gpu_moments) on the GPU against
+the oracle's restatements."""
+import numpy as np
+import pytest
+
+from ps3d_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+PI = np.pi
+
+
+@pytest.fixture(scope="module")
+def G():
+    from ps3d_amd import gpu
+    gpu.lib()
+    return gpu
+
+
+CASES = [
+    (32, 32, 32, (2 * PI, 2 * PI, PI)),
+    (48, 24, 16, (2 * PI, 2 * PI, PI)),
+    (40, 30, 20, (3.0, 2.0, 1.25)),      # factors of 5, unequal extents (dk > 1)
+    (64, 64, 128, (2 * PI, 2 * PI, 2 * PI)),
+    (16, 16, 64, (16.0, 16.0, 4.0)),     # dk < 1: shells beyond kmax exist and are dropped
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%dx%dx%d" % c[:3])
+def test_spectrum_matches_oracle(oracle_mod, G, case):
+    nx, ny, nz, ext = case
+    print("spectrum", case, flush=True)
+    kw = dict(lower=(0.0, 0.0, 0.0), extent=ext)
+    o = oracle_mod.Oracle(oracle_mod.make_params(nx, ny, nz, **kw))
+    d = G.Ps3dGpu(G.make_params(nx, ny, nz, **kw))
+    vel, _ = ics.random_phase_fields(nx, ny, nz, k0=3.0, seed=nx + nz, extent=ext)
+    kg, sg, ng = d.spectrum(vel)
+    ko, so, no = oracle_mod.spectrum(o, vel)
+    assert len(sg) == len(so)
+    assert np.array_equal(ng, no)
+    assert np.array_equal(kg, ko)
+    assert np.abs(sg - so).max() <= 1e-12 * np.abs(so).max()
+    d.close()
+    o.close()
+
+
+def test_spectrum_of_the_state_and_moments(oracle_mod, G):
+    """vel = NULL bins the velocity of the last vor2vel; the moments match the
+    oracle's, and the helicity is the adapt record's."""
+    n = 32
+    kw = dict(lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI))
+    d = G.Ps3dGpu(G.make_params(n, n, n, **kw))
+    d.set_vorticity(ics.beltrami_multimode(n, n, n, seed=2))
+    d.advance_n(2)
+    d.vor2vel()
+    _, diag = d.adapt(d.t)
+    vel, vor = d.get(G.VEL), d.get(G.VOR)
+    m = d.moments()
+    mo = oracle_mod.moments(vel, vor)
+    for key in mo:
+        assert abs(m[key] - mo[key]) <= 1e-12 * max(abs(mo[key]), 1e-300), key
+    assert abs(m["helicity"] - diag[G.DIAG_NAMES.index("hel")]) <= 1e-12 * abs(m["helicity"])
+    k1, s1, n1 = d.spectrum()
+    o = oracle_mod.Oracle(oracle_mod.make_params(n, n, n, **kw))
+    ko, so, no = oracle_mod.spectrum(o, vel)
+    assert np.array_equal(n1, no)
+    assert np.abs(s1 - so).max() <= 1e-12 * np.abs(so).max()
+    # the same velocity handed in gives the same spectrum, bit for bit
+    k2, s2, n2 = d.spectrum(vel)
+    assert np.array_equal(s1, s2) and np.array_equal(n1, n2)
+    # the handle's velocity was replaced: stepping needs a vor2vel first
+    with pytest.raises(G.Ps3dError):
+        d.adapt(d.t)
+    d.close()
+    o.close()
+
+
+def test_spectrum_refuses_a_short_buffer(G):
+    import ctypes as C
+    d = G.Ps3dGpu(G.make_params(16, 16, 16, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI)))
+    ns, dk = C.c_int(), C.c_double()
+    spec = np.zeros(4)
+    st = G.lib().ps3d_gpu_spectrum(d.h, None, 4, C.byref(ns), C.byref(dk), spec.ctypes.data, None)
+    assert st == 1 and ns.value == 21
+    d.close()
+
+
+def test_spectrum_at_256(G):
+    """Full-size grid: the spectrum integrates to the kinetic energy of field_stats
+    and the counts cover every slot once."""
+    n = 256
+    d = G.Ps3dGpu(G.make_params(n, n, n, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI),
+                                filtering="2/3-rule", nnu=3, prediss=30.0, pretype="vorch"))
+    d.set_vorticity(ics.beltrami_multimode(n, n, n, seed=1))
+    d.vor2vel()
+    print("spectrum 256", flush=True)
+    k, s, num = d.spectrum()
+    ke = d.moments()["ke"]
+    assert num.sum() == n * n * (n + 1)
+    assert abs(np.sum(s) * k[1] - ke) <= 1e-12 * ke
+    d.close()

@@ -267,7 +267,7 @@ def test_derived_pressure_divergence(oracle_mod, G, n, nsteps):
     d.close()
 
 
-@pytest.mark.parametrize("shape", [(16, 12, 128), (8, 16, 256), (4, 6, 512), (4, 4, 768), (4, 4, 1024)])
+@pytest.mark.parametrize("shape", [(24, 20, 64), (16, 12, 128), (8, 16, 256), (4, 6, 512), (4, 4, 768), (4, 4, 1024)])
 @pytest.mark.parametrize("filtering", ["Hou & Li", "2/3-rule"])
 def test_long_column_trajectory(oracle_mod, G, shape, filtering):
     """Adaptive steps on grids whose z columns take the wave-per-column
@@ -292,13 +292,43 @@ def test_long_column_trajectory(oracle_mod, G, shape, filtering):
     o.close()
 
 
+@pytest.mark.parametrize("shape", [(40, 30, 20), (50, 20, 40), (32, 80, 100), (160, 10, 50)])
+def test_radix5_grid_trajectory(oracle_mod, G, shape):
+    """Grids whose lengths carry factors of 5 (stafft's factorisen accepts
+    2^a 3^b 5^c, stafft.f90:128-187; radix-5 passes forrdx5/revrdx5): adaptive
+    steps against the oracle, fields and diagnostics within 1e-12."""
+    nx, ny, nz = shape
+    kw = dict(filtering="Hou & Li", stepper="impl-diff-rk4", nnu=3, prediss=30.0, pretype="vorch",
+              length_scale="Kolmogorov", alpha=0.1, dt_fixed=0.0, limit=1.0e30)
+    g = dict(nx=nx, ny=ny, nz=nz, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI))
+    o = oracle_mod.Oracle(oracle_mod.make_params(**g, **kw))
+    d = G.Ps3dGpu(G.make_params(**g, **kw))
+    vor = ics.beltrami_multimode(nx, ny, nz, seed=4)
+    o.set_vorticity(vor)
+    d.set_vorticity(vor)
+    print("radix 5", shape, flush=True)
+    do = o.advance_n(5)
+    dd = d.advance_n(5)
+    r = diag_rel(dd, do, DIAG_COLS)
+    assert max(r.values()) < TOL, r
+    assert field_rel(d.get(G.SVOR), o.get(oracle_mod.SVOR)) < TOL
+    d.close()
+    o.close()
+
+
+def test_grid_with_prime_factor_7_is_refused(G):
+    with pytest.raises(G.Ps3dError, match="Factorisation"):
+        G.Ps3dGpu(G.make_params(**dict(box(28), nz=16), **config1()))
+
+
+@pytest.mark.parametrize("nz", [64, 256])
 @pytest.mark.parametrize("variant", ["1", "2", "3"])
-def test_wave_and_quad_z_kernels_agree(G, monkeypatch, variant):
+def test_wave_and_quad_z_kernels_agree(G, monkeypatch, variant, nz):
     """The wave-per-column (both register budgets) and quad-per-thread z kernels
     give the same step."""
     p = G.make_params(**box(32), **config1())
-    p.nz = 256
-    vor = ics.beltrami_multimode(32, 32, 256, seed=5)
+    p.nz = nz
+    vor = ics.beltrami_multimode(32, 32, nz, seed=5)
     monkeypatch.setenv("PS3D_ZWAVE", "0")
     a = G.Ps3dGpu(p)
     monkeypatch.setenv("PS3D_ZWAVE", variant)
@@ -361,5 +391,31 @@ def test_step_graph_replays_equal_eager_launches(G, monkeypatch):
             m.vor2vel()
             m.derived()
         assert np.array_equal(a.get(G.PRES), b.get(G.PRES))
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("shape", [(32, 48, 16), (64, 96, 16), (128, 256, 8)])
+@pytest.mark.parametrize("sched", ["0,0", "1,1", "2,2", "3,3"])
+def test_line_fft_schedules_agree(oracle_mod, G, monkeypatch, shape, sched):
+    """Every FFT schedule of the transform-only line passes (PS3D_LINE_SCHED = x,y:
+    radix-8 or radix-16 first plan, Hermitian pair folded or in its own pass) gives
+    the default schedule's step and the oracle's."""
+    nx, ny, nz = shape
+    kw = dict(nx=nx, ny=ny, nz=nz, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI))
+    cfg = config1(dt_fixed=0.0)
+    vor = ics.beltrami_multimode(nx, ny, nz, seed=9)
+    a = G.Ps3dGpu(G.make_params(**kw, **cfg))
+    monkeypatch.setenv("PS3D_LINE_SCHED", sched)
+    b = G.Ps3dGpu(G.make_params(**kw, **cfg))
+    o = oracle_mod.Oracle(oracle_mod.make_params(**kw, **cfg))
+    for m in (a, b, o):
+        m.set_vorticity(vor)
+    print("sched", sched, shape, flush=True)
+    da, db, do = a.advance_n(2), b.advance_n(2), o.advance_n(2)
+    assert field_rel(b.get(G.SVOR), a.get(G.SVOR)) < 1e-13
+    assert field_rel(b.get(G.SVOR), o.get(oracle_mod.SVOR)) < TOL
+    r = diag_rel(db, do, DIAG_COLS)
+    assert max(r.values()) < TOL, r
     a.close()
     b.close()

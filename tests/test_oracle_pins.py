@@ -42,7 +42,8 @@ def dst_matrix(n):
     return np.sqrt(2.0 / n) * np.sin(PI * ((k * j) % (2 * n)) / n)
 
 
-@pytest.mark.parametrize("n", [8, 16, 32, 48, 64, 96, 128, 256])
+# 10, 20, 40, 50, 100, 250: stafft's radix-5 passes (forrdx5/revrdx5, stafft.f90:754-897, 1355-1499)
+@pytest.mark.parametrize("n", [8, 10, 16, 20, 32, 40, 48, 50, 64, 96, 100, 128, 250, 256])
 def test_forfft_revfft_bruteforce(oracle_mod, n):
     rng = np.random.default_rng(n)
     x = rng.standard_normal((n, 7))
@@ -52,7 +53,7 @@ def test_forfft_revfft_bruteforce(oracle_mod, n):
     assert np.abs(back - x).max() < 1e-14
 
 
-@pytest.mark.parametrize("n", [8, 16, 32, 48, 64, 96, 128])
+@pytest.mark.parametrize("n", [8, 10, 16, 20, 32, 40, 48, 50, 64, 96, 100, 128])
 def test_dct_dst_bruteforce(oracle_mod, n):
     rng = np.random.default_rng(100 + n)
     y = rng.standard_normal((n + 1, 5))
@@ -71,7 +72,7 @@ def test_dct_dst_bruteforce(oracle_mod, n):
     assert np.abs(oracle_mod.fft1d("dst", s) - zz).max() < 1e-13
 
 
-@pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128, 256, 512])
+@pytest.mark.parametrize("n", [16, 20, 32, 40, 48, 60, 64, 80, 96, 100, 128, 160, 250, 256, 320, 500, 512])
 def test_transforms_match_reference_build(oracle_mod, n):
     if not oracle_mod.ref_available():
         pytest.skip("oracle/_ref not built (needs /root/reference + amdflang)")
