@@ -337,7 +337,8 @@ int ps3d_gpu_ensemble_line_choice(ps3d_ensemble *e, int g, int *threads);
 /* FFT schedules the tuner chose for group g's transform-only line passes:
  * sched[0] x lines, sched[1] y lines; bit 1 = radix-8-first plan (else
  * radix-16-first), bit 2 = the Hermitian pair of the inverse formed in the first
- * radix pass (else a pass of its own).  No counterpart in the reference. */
+ * radix pass (else a pass of its own), bit 4 = 512 threads per tile (else 256).
+ * No counterpart in the reference. */
 int ps3d_gpu_ensemble_fft_schedule(ps3d_ensemble *e, int g, int *sched);
 int ps3d_gpu_ensemble_destroy(ps3d_ensemble *e);
 

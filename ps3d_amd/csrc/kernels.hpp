@@ -206,11 +206,12 @@ struct LineArgs {
     uint16_t *nn_in;   // LM_XMAIN: also write the closure input (bf16 [p][16]) when set
     float nn_scale[6];
     int rthreads;      // register-holding modes: 1024 (standard tile), 512 (half depth) or 256 (quarter)
-    int sched;         // transform-only modes: FFT schedule (LS_R8 | LS_FOLD bits)
+    int sched;         // transform-only modes: FFT schedule (LS_R8 | LS_FOLD | LS_T512 bits)
 };
 // FFT schedules of the transform-only line passes: radix-8-first plan instead of
-// radix-16-first; Hermitian pair of the inverse formed in the first radix pass.
-enum { LS_R8 = 1, LS_FOLD = 2 };
+// radix-16-first; Hermitian pair of the inverse formed in the first radix pass; 512
+// threads per tile instead of 256.
+enum { LS_R8 = 1, LS_FOLD = 2, LS_T512 = 4 };
 // the schedule a line length takes unless told otherwise (fold from n = 128 up)
 int line_sched_default(int n);
 // whether a schedule is built for a line length (others fall back to the default)

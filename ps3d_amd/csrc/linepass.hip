@@ -493,10 +493,12 @@ __device__ inline double eig_absmax_est(double s11, double s12, double s13, doub
 // Transform-only modes (forward, inverse, derivative) of nf fields; the global
 // loads of field f+1 are issued before the FFT of field f.  SCHED: the FFT
 // schedule, bit LS_R8 = radix-8-first plan instead of radix-16-first, bit LS_FOLD
-// = Hermitian pair of the inverse formed in the first radix pass.
+// = Hermitian pair of the inverse formed in the first radix pass, bit LS_T512 = 512
+// threads per tile instead of 256 (twice the waves on the same LDS).
 template <int N, int SCHED>
-__global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
+__global__ void __launch_bounds__((SCHED & LS_T512) ? 512 : 256) k_line_simple(LineArgs a) {
     constexpr bool R8 = (SCHED & LS_R8) != 0, FOLD = (SCHED & LS_FOLD) != 0;
+    constexpr int T = (SCHED & LS_T512) ? 512 : 256;
     extern __shared__ double2 lsmem[];
     if (gated_off(a.g.gate)) return;
     Tile t = make_tile(a);
@@ -530,13 +532,13 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
         // deep tiles load level pairs (16 bytes), shallower ones single levels
         constexpr int ZC = zc_for(N), NL = ZC / 2;
         constexpr bool PAIR = ZC >= 16;
-        constexpr int PER = PAIR ? (N * NL + 255) / 256 : (N * ZC + 255) / 256;
+        constexpr int PER = PAIR ? (N * NL + T - 1) / T : (N * ZC + T - 1) / T;
         double2 pre[PAIR ? PER : 1];
         double pre1[PAIR ? 1 : PER];
         auto gload = [&](const double *__restrict__ src) {
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
-                const int e = threadIdx.x + k * 256;
+                const int e = threadIdx.x + k * T;
                 if constexpr (PAIR) {
                     const int q = e % NL, j = e / NL;
                     pre[k] = j < N ? gpair(t, a, src, j, q) : make_double2(0.0, 0.0);
@@ -550,7 +552,7 @@ __global__ void __launch_bounds__(256) k_line_simple(LineArgs a) {
         for (int f = 0; f < a.nf; ++f) {
 #pragma unroll
             for (int k = 0; k < PER; ++k) {
-                const int e = threadIdx.x + k * 256;
+                const int e = threadIdx.x + k * T;
                 if constexpr (PAIR) {
                     const int q = e % NL, j = e / NL;
                     if (j < N) A[(size_t)q * t.LS + t.sl(j)] = pre[k];
@@ -982,7 +984,8 @@ hipError_t configure_linepass() {
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_reg<N_, LM_XADAPT, T_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 #define CFGS(N_, S_)                                                                                         \
     if (e == hipSuccess) e = hipFuncSetAttribute((const void *)k_line_simple<N_, S_>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-#define CFG(N_) CFGS(N_, 0) CFGS(N_, 1) CFGS(N_, 2) CFGS(N_, 3) CFGT(N_, kLineThreads)
+#define CFG(N_) CFGS(N_, 0) CFGS(N_, 1) CFGS(N_, 2) CFGS(N_, 3) CFGS(N_, 4) CFGS(N_, 5) CFGS(N_, 6) CFGS(N_, 7) \
+    CFGT(N_, kLineThreads)
 #define CFG2(N_) CFG(N_) CFGT(N_, 512)
 #define CFG4(N_) CFGT(N_, 256)
     CFGS(0, 0)
@@ -1018,11 +1021,11 @@ static void launch_line_t(const LineArgs &a, hipStream_t stream) {
         if constexpr (N == 0) {
             hipLaunchKernelGGL((k_line_simple<0, 0>), dim3(blocks, nb), dim3(256), lds, stream, a);
         } else {
-            switch (a.sched & 3) {
-                case 0: hipLaunchKernelGGL((k_line_simple<N, 0>), dim3(blocks, nb), dim3(256), lds, stream, a); break;
-                case 1: hipLaunchKernelGGL((k_line_simple<N, 1>), dim3(blocks, nb), dim3(256), lds, stream, a); break;
-                case 2: hipLaunchKernelGGL((k_line_simple<N, 2>), dim3(blocks, nb), dim3(256), lds, stream, a); break;
-                default: hipLaunchKernelGGL((k_line_simple<N, 3>), dim3(blocks, nb), dim3(256), lds, stream, a); break;
+            switch (a.sched & 7) {
+#define SCH(S_) \
+    case S_: hipLaunchKernelGGL((k_line_simple<N, S_>), dim3(blocks, nb), dim3((S_ & LS_T512) ? 512 : 256), lds, stream, a); break;
+                SCH(0) SCH(1) SCH(2) SCH(3) SCH(4) SCH(5) SCH(6) SCH(7)
+#undef SCH
             }
         }
     }
@@ -1051,10 +1054,14 @@ bool line_reg_ok(int n, int T) {
 
 int line_reg_zc(int n, int T) { return zc_reg(n, T); }
 
-int line_sched_default(int n) { return n >= 128 ? LS_FOLD : 0; }
+// Measured on the plain workload (steps/s under each schedule of the y passes):
+// 128^3 and 512^3 run fastest with the fold at 256 threads, 256^3 with 512 threads
+// and the pair pass (116.5 against 115.6 steps/s); the small grids of the ensemble
+// go to the tuner.
+int line_sched_default(int n) { return n == 256 ? LS_T512 : (n >= 128 ? LS_FOLD : 0); }
 
 bool line_sched_ok(int n, int sched) {
-    if (sched < 0 || sched > 3) return false;
+    if (sched < 0 || sched > 7) return false;
     bool built = false, r8 = false;
     switch (n) {
 #define CASE(N_) case N_: built = true; r8 = CPlan8<N_>::ok; break;

@@ -2055,7 +2055,7 @@ int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice) {
             int &cur = axis == 0 ? h->lsx : h->lsy;
             int bests = cur;
             float bt = 1e30f;
-            for (int sc = 0; sc < 4; ++sc) {
+            for (int sc = 0; sc < 8; ++sc) {
                 if (!line_sched_ok(n, sc)) continue;
                 cur = sc;
                 if ((st = enqueue_steps(h, 1))) break;

@@ -51,6 +51,12 @@ struct WDim {
     static constexpr int P = NZ / 32;      // complex points per lane of a length-2nz line
     static constexpr int IPL = (8 * P + 63) / 64;   // 8-point DFT items per lane across lanes
     static constexpr int WS = 72 * P;      // double2 of LDS per wave
+    // nz <= 256: the exchanges of vor2vel's first and third phase go to a second LDS
+    // region (two fields per member) beside the wave slices, so that no exchange
+    // needs a barrier after its reads (the next write to a region comes after the
+    // next exchange's barrier); nz = 512 keeps one region (the LDS of two blocks
+    // per CU would not hold a second one)
+    static constexpr bool DB = NZ <= 256;
 };
 
 // One field of one column: z = lane + 64 s in v[s], z = nz in top (same in all lanes).
@@ -171,26 +177,39 @@ struct WExp {
     double ep[WDim<NZ>::S + 1], em[WDim<NZ>::S + 1];
     double ef, div;
 };
+// The column's exponentials and division, formed once per kernel (a kernel that
+// runs both the tendency and the vor2vel pass expands them three times).
+struct WSeed {
+    double e, f, e64, ef, div;
+};
+__device__ inline WSeed make_wseed(const WC &c, const Grid &g) {
+    WSeed x;
+    const double kd = c.K * g.dz;
+    x.e64 = exp(-64.0 * kd);
+    x.ef = exp(-c.K * g.lz);
+    x.div = 1.0 / (1.0 - x.ef * x.ef);
+    x.e = exp(-kd * (double)c.lane);
+    x.f = exp(-kd * (double)(64 - c.lane));   // z = lane + 64 (S - 1): zm = dz (64 - lane)
+    return x;
+}
 template <int NZ>
-__device__ inline WExp<NZ> make_wexp(const WC &c, const Grid &g) {
+__device__ inline WExp<NZ> make_wexp(const WSeed &sd) {
     constexpr int S = WDim<NZ>::S;
     static_assert(NZ == 64 * S, "wave layout covers nz = 64 S");
     WExp<NZ> x;
-    const double kd = c.K * g.dz;
-    const double e64 = exp(-64.0 * kd);
-    x.ef = exp(-c.K * g.lz);
-    x.div = 1.0 / (1.0 - x.ef * x.ef);
-    double e = exp(-kd * (double)c.lane);
+    x.ef = sd.ef;
+    x.div = sd.div;
+    double e = sd.e;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         x.ep[s] = e;
-        e *= e64;
+        e *= sd.e64;
     }
-    double f = exp(-kd * (double)(64 - c.lane));   // z = lane + 64 (S - 1): zm = dz (64 - lane)
+    double f = sd.f;
 #pragma unroll
     for (int s = S - 1; s >= 0; --s) {
         x.em[s] = f;
-        f *= e64;
+        f *= sd.e64;
     }
     x.ep[S] = x.ef;
     x.em[S] = 1.0;
@@ -203,9 +222,9 @@ struct WPhi {
     double pm[WDim<NZ>::S], pp[WDim<NZ>::S];
 };
 template <int NZ>
-__device__ inline WPhi<NZ> make_phi(const WC &c, const Grid &g) {
+__device__ inline WPhi<NZ> make_phi(const WC &c, const Grid &g, const WSeed &sd) {
     WPhi<NZ> h;
-    const WExp<NZ> x = make_wexp<NZ>(c, g);
+    const WExp<NZ> x = make_wexp<NZ>(sd);
 #pragma unroll
     for (int s = 0; s < WDim<NZ>::S; ++s) {
         if (c.q00) {
@@ -417,21 +436,25 @@ __device__ inline void wxform(const WC &c, WF<NZ> &F0, WF<NZ> &F1, double2 *ws, 
 }
 
 // LDS exchange between the four member waves: each wave publishes up to three
-// fields in its own slice; readers take member m' after the barrier.
-template <int NZ>
+// fields in its own slice (BUF = 0) or two in the second region (BUF = 1); readers
+// take member m' after the barrier.
+template <int NZ, int BUF>
 __device__ inline double *xslot(double2 *wsm, int member, int field) {
-    return reinterpret_cast<double *>(wsm + (size_t)member * WDim<NZ>::WS) + field * (NZ + 1);
+    if constexpr (BUF == 0)
+        return reinterpret_cast<double *>(wsm + (size_t)member * WDim<NZ>::WS) + field * (NZ + 1);
+    else
+        return reinterpret_cast<double *>(wsm + (size_t)4 * WDim<NZ>::WS) + (member * 2 + field) * (NZ + 1);
 }
-template <int NZ>
+template <int NZ, int BUF = 0>
 __device__ inline void xput(const WC &c, double2 *wsm, int field, const WF<NZ> &F) {
-    double *p = xslot<NZ>(wsm, c.m, field);
+    double *p = xslot<NZ, BUF>(wsm, c.m, field);
 #pragma unroll
     for (int s = 0; s < WDim<NZ>::S; ++s) p[c.lane + 64 * s] = F.v[s];
     if (c.lane == 0) p[NZ] = F.top;
 }
-template <int NZ>
+template <int NZ, int BUF = 0>
 __device__ inline void xget(const WC &c, double2 *wsm, int member, int field, WF<NZ> &F) {
-    const double *p = xslot<NZ>(wsm, member, field);
+    const double *p = xslot<NZ, BUF>(wsm, member, field);
 #pragma unroll
     for (int s = 0; s < WDim<NZ>::S; ++s) F.v[s] = p[c.lane + 64 * s];
     F.top = p[NZ];
@@ -578,7 +601,7 @@ __device__ inline void wzero(WF<NZ> &F) {
 // this wave's registers (the fused tendency + next-stage pass below).
 template <int NZ>
 __device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &a, const WC &c, double2 *wsm,
-                                                                const WF<NZ> *reg) {
+                                                                const WF<NZ> *reg, const WSeed &sd) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     const Grid g = a.g;
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
@@ -599,20 +622,22 @@ __device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &
     }
 
     // D = B_x - A_y (inversion.f90:41-45)
-    xput(c, wsm, 0, Et);
-    xput(c, wsm, 1, Xi);
+    constexpr bool DB = WDim<NZ>::DB;
+    constexpr int XB = DB ? 1 : 0;
+    xput<NZ, XB>(c, wsm, 0, Et);
+    xput<NZ, XB>(c, wsm, 1, Xi);
     __syncthreads();
     {
         WF<NZ> t1, t2;
-        xget(c, wsm, m1, 0, t1);
-        xget(c, wsm, m2, 1, t2);
+        xget<NZ, XB>(c, wsm, m1, 0, t1);
+        xget<NZ, XB>(c, wsm, m2, 1, t2);
 #pragma unroll
         for (int s = 0; s < S; ++s) D.v[s] = c.fx * t1.v[s] - c.fy * t2.v[s];
         D.top = c.fx * t1.top - c.fy * t2.top;
     }
-    __syncthreads();
+    if constexpr (!DB) __syncthreads();   // the slices are the FFTs' next
 
-    const WPhi<NZ> ph = make_phi<NZ>(c, g);
+    const WPhi<NZ> ph = make_phi<NZ>(c, g, sd);
     // C = combine(zeta); E = decompose(central_diffz(C)) (46-48)
     {
         const double c0 = wlo(C), cn = C.top;
@@ -649,7 +674,7 @@ __device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &
             Ep.top = c.k2l2i * (c.fy * e2.top - c.fx * d1.top);
         }
     }
-    __syncthreads();
+    if constexpr (!DB) __syncthreads();   // the next exchange writes the same slots
     if (a.svor_out[0]) {
         wstore(c, a.svor_out[0], Xp);
         wstore(c, a.svor_out[1], Ep);
@@ -657,25 +682,25 @@ __device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &
 
     // source of w: ds = A'_y - B'_x (86-90)
     WF<NZ> Ds;
-    xput(c, wsm, 0, Xp);
-    xput(c, wsm, 1, Ep);
+    xput<NZ, XB>(c, wsm, 0, Xp);
+    xput<NZ, XB>(c, wsm, 1, Ep);
     __syncthreads();
     {
         WF<NZ> x2, e1;
-        xget(c, wsm, m2, 0, x2);
-        xget(c, wsm, m1, 1, e1);
+        xget<NZ, XB>(c, wsm, m2, 0, x2);
+        xget<NZ, XB>(c, wsm, m1, 1, e1);
 #pragma unroll
         for (int s = 0; s < S; ++s) Ds.v[s] = c.fy * x2.v[s] - c.fx * e1.v[s];
         Ds.top = c.fy * x2.top - c.fx * e1.top;
     }
-    __syncthreads();
+    if constexpr (!DB) __syncthreads();   // the slices are the FFTs' next
     const double d0 = wlo(Ds), dn = Ds.top;
     const double xp0 = wlo(Xp), xpn = Xp.top, ep0 = wlo(Ep), epn = Ep.top;
 
     // bs (interior) and es (all z): boundary parts (94-104); green and d/dz of the
     // sine part (107-122); quad (0,0): the mean flow (150-165) in members 1, 2
     WF<NZ> Bs, As;
-    const WExp<NZ> wx = make_wexp<NZ>(c, g);
+    const WExp<NZ> wx = make_wexp<NZ>(sd);
 #pragma unroll
     for (int s = 0; s <= S; ++s) {
         const int z = s < S ? l + 64 * s : NZ;
@@ -787,7 +812,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     extern __shared__ double2 wsm[];
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
-    zinv_body<NZ>(a, c, wsm, nullptr);
+    zinv_body<NZ>(a, c, wsm, nullptr, make_wseed(c, a.g));
 }
 
 // ---------------------------------------------------------------------------
@@ -798,7 +823,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
 // next stage's vor2vel pass instead of going to memory.
 template <int NZ, bool KEEP>
 __device__ __attribute__((always_inline)) inline void ztend_body(const ZTendArgs &a, const WC &c, double2 *wsm,
-                                                                 WF<NZ> *keep) {
+                                                                 WF<NZ> *keep, const WSeed &sd) {
     constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
     const Grid g = a.g;
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
@@ -814,7 +839,7 @@ __device__ __attribute__((always_inline)) inline void ztend_body(const ZTendArgs
     wdiff(c, g, Pp, FP);
     // field_decompose of r, q, p, dq/dz, dp/dz: subtract the harmonic part, then DST
     {
-        const WPhi<NZ> ph = make_phi<NZ>(c, g);
+        const WPhi<NZ> ph = make_phi<NZ>(c, g, sd);
         const double r0 = wlo(R), rn = R.top, q0 = wlo(Q), qn = Q.top, p0 = wlo(Pp), pn = Pp.top;
         const double fq0 = wlo(FQ), fqn = FQ.top, fp0 = wlo(FP), fpn = FP.top;
         wharm(c, ph, R, r0, rn, -1.0);
@@ -885,13 +910,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
     extern __shared__ double2 wsm[];
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
-    ztend_body<NZ, false>(a, c, wsm, nullptr);
+    ztend_body<NZ, false>(a, c, wsm, nullptr, make_wseed(c, a.g));
 }
 
 // RK4 stage s (1..3) of the tendency pass and the vor2vel z pass of stage s+1 in
 // one kernel: the new svor of each column never leaves the registers of the wave
 // that formed it (saves writing and re-reading three volumes per stage boundary).
-// One barrier separates the two partner exchanges' uses of the block's LDS.
+// At nz = 512 one barrier separates the two partner exchanges' uses of the block's
+// LDS.
 template <int NZ, int OCC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
 k_ztend_zinv_w(ZTendArgs a, ZInvArgs b) {
@@ -899,9 +925,12 @@ k_ztend_zinv_w(ZTendArgs a, ZInvArgs b) {
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
     WF<NZ> q[3];
-    ztend_body<NZ, true>(a, c, wsm, q);
-    __syncthreads();
-    zinv_body<NZ>(b, c, wsm, q);
+    const WSeed sd = make_wseed(c, a.g);
+    ztend_body<NZ, true>(a, c, wsm, q, sd);
+    // (with the second region the vor2vel pass's first exchange does not touch the
+    // slices the tendency pass's exchange used)
+    if constexpr (!WDim<NZ>::DB) __syncthreads();
+    zinv_body<NZ>(b, c, wsm, q, sd);
 }
 
 // Single z operators on one volume (kernel checks of the wave path).
@@ -931,7 +960,7 @@ __global__ void __launch_bounds__(256) k_zop_w(ZOpArgs a) {
             wstore(c, a.out, O);
             break;
         case ZOP_COMBINE: {
-            const WPhi<NZ> ph = make_phi<NZ>(c, g);
+            const WPhi<NZ> ph = make_phi<NZ>(c, g, make_wseed(c, g));
             const double lo = wlo(F), hi = F.top;
             wxform<NZ, WK_DST, WK_NONE>(c, F, none, ws, tw, sc);
             wharm(c, ph, F, lo, hi, 1.0);
@@ -939,7 +968,7 @@ __global__ void __launch_bounds__(256) k_zop_w(ZOpArgs a) {
             break;
         }
         case ZOP_DECOMPOSE: {
-            const WPhi<NZ> ph = make_phi<NZ>(c, g);
+            const WPhi<NZ> ph = make_phi<NZ>(c, g, make_wseed(c, g));
             wharm(c, ph, F, wlo(F), F.top, -1.0);
             wxform<NZ, WK_DST, WK_NONE>(c, F, none, ws, tw, sc);
             wstore(c, a.out, F);
@@ -971,7 +1000,10 @@ __global__ void __launch_bounds__(256) k_zop_w(ZOpArgs a) {
 // ---------------------------------------------------------------------------
 bool zwave_supported(int nz) { return nz == 64 || nz == 128 || nz == 256 || nz == 512; }
 
-size_t zwave_lds_bytes(int nz) { return 4 * (size_t)72 * (size_t)(nz / 32) * sizeof(double2); }
+size_t zwave_lds_bytes(int nz) {
+    // four wave slices, and for nz <= 256 the second exchange region
+    return 4 * (size_t)72 * (size_t)(nz / 32) * sizeof(double2) + (nz <= 256 ? (size_t)8 * (nz + 1) * sizeof(double) : 0);
+}
 
 #define PS3D_ZW_SIZES(X) X(64) X(128) X(256) X(512)
 

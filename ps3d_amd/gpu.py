@@ -410,12 +410,13 @@ class Ps3dEnsemble:
     def fft_schedules(self):
         """(x lines, y lines) FFT schedule of each group's transform-only passes as
         names: radix-16 or radix-8 first plan, Hermitian pair folded into the first
-        pass or formed in a pass of its own."""
+        pass or formed in a pass of its own, +t512 for 512 threads per tile."""
         out = []
         for g in range(len(self.groups)):
             t = (C.c_int * 2)()
             _check(lib().ps3d_gpu_ensemble_fft_schedule(self.e, g, t))
-            out.append(tuple(("r8" if v & 1 else "r16") + ("+fold" if v & 2 else "+pairpass") for v in t))
+            out.append(tuple(("r8" if v & 1 else "r16") + ("+fold" if v & 2 else "+pairpass") + ("+t512" if v & 4 else "")
+                             for v in t))
         return out
 
     def set_choice(self, g, code):

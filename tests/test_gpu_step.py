@@ -396,11 +396,11 @@ def test_step_graph_replays_equal_eager_launches(G, monkeypatch):
 
 
 @pytest.mark.parametrize("shape", [(32, 48, 16), (64, 96, 16), (128, 256, 8)])
-@pytest.mark.parametrize("sched", ["0,0", "1,1", "2,2", "3,3"])
+@pytest.mark.parametrize("sched", ["0,0", "1,1", "2,2", "3,3", "4,5", "6,7"])
 def test_line_fft_schedules_agree(oracle_mod, G, monkeypatch, shape, sched):
     """Every FFT schedule of the transform-only line passes (PS3D_LINE_SCHED = x,y:
-    radix-8 or radix-16 first plan, Hermitian pair folded or in its own pass) gives
-    the default schedule's step and the oracle's."""
+    radix-8 or radix-16 first plan, Hermitian pair folded or in its own pass, 256 or
+    512 threads per tile) gives the default schedule's step and the oracle's."""
     nx, ny, nz = shape
     kw = dict(nx=nx, ny=ny, nz=nz, lower=(0.0, 0.0, 0.0), extent=(2 * PI, 2 * PI, PI))
     cfg = config1(dt_fixed=0.0)
