@@ -137,11 +137,14 @@ __device__ inline void get_lids(const ZT<ZPT> &T, const ZLds &L, const double (&
 #pragma unroll
     for (int s = 0; s < ZPT; ++s) {
         if (T.has[s] && T.z[s] == 0)
+#pragma unroll
             for (int m = 0; m < 4; ++m) lid[m] = F[s][m];
         if (T.has[s] && T.z[s] == T.nz)
+#pragma unroll
             for (int m = 0; m < 4; ++m) lid[4 + m] = F[s][m];
     }
     __syncthreads();
+#pragma unroll
     for (int m = 0; m < 4; ++m) {
         lo[m] = lid[m];
         hi[m] = lid[4 + m];
@@ -233,12 +236,14 @@ __device__ inline void zdiff(const ZT<ZPT> &T, const ZLds &L, const Grid &g, con
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
         if (T.has[s])
+#pragma unroll
             for (int m = 0; m < 4; ++m) cb[m * nz1 + T.z[s]] = F[s][m];
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < ZPT; ++s) {
         if (!T.has[s]) continue;
         const int z = T.z[s];
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             const double *c = cb + m * nz1;
             if (z == 0)
@@ -263,6 +268,7 @@ __device__ inline void zdecompose(const ZT<ZPT> &T, const ZLds &L, const Grid &g
         if (!T.has[s] || T.z[s] == 0 || T.z[s] == T.nz) continue;
         double pm, pp;
         phi_only(g, T.K, T.q00, T.z[s], pm, pp);
+#pragma unroll
         for (int m = 0; m < 4; ++m) F[s][m] = F[s][m] - (lo[m] * pm + hi[m] * pp);
     }
     zdst4<ZPT, NZ>(T, L, F, P, tw, scale);
@@ -280,6 +286,7 @@ __device__ inline void zcombine(const ZT<ZPT> &T, const ZLds &L, const Grid &g, 
         if (!T.has[s] || T.z[s] == 0 || T.z[s] == T.nz) continue;
         double pm, pp;
         phi_only(g, T.K, T.q00, T.z[s], pm, pp);
+#pragma unroll
         for (int m = 0; m < 4; ++m) F[s][m] = F[s][m] + lo[m] * pm + hi[m] * pp;
     }
 }
@@ -385,6 +392,7 @@ __device__ inline void get_lids3(const ZT<ZPT> &T, const ZLds &L, const double (
         if (!T.has[s]) continue;
         const int off = T.z[s] == 0 ? 0 : (T.z[s] == T.nz ? 12 : -1);
         if (off < 0) continue;
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             lid[off + m] = F0[s][m];
             lid[off + 4 + m] = F1[s][m];
@@ -393,6 +401,7 @@ __device__ inline void get_lids3(const ZT<ZPT> &T, const ZLds &L, const double (
     }
     __syncthreads();
     for (int f = 0; f < 3; ++f)
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             lo[f][m] = lid[4 * f + m];
             hi[f][m] = lid[12 + 4 * f + m];
@@ -410,6 +419,7 @@ __device__ inline void fd_from_cols(const ZT<ZPT> &T, const Grid &g, const doubl
     for (int s = 0; s < ZPT; ++s) {
         if (!T.has[s]) continue;
         const int z = T.z[s];
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             const double *c = cb + m * nz1;
             if (z == 0)
@@ -420,6 +430,7 @@ __device__ inline void fd_from_cols(const ZT<ZPT> &T, const Grid &g, const doubl
                 D[s][m] = (c[z + 1] - c[z - 1]) * g.hdzi;
         }
     }
+#pragma unroll
     for (int m = 0; m < 4; ++m) {
         const double *c = cb + m * nz1;
         lo[m] = g.dzi * (c[1] - c[0]);
@@ -432,6 +443,7 @@ __device__ inline void put_cols(const ZT<ZPT> &T, double *cb, const double (&F)[
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
         if (T.has[s])
+#pragma unroll
             for (int m = 0; m < 4; ++m) cb[m * nz1 + T.z[s]] = F[s][m];
 }
 template <int ZPT>
@@ -468,6 +480,7 @@ __device__ inline void harmonic(const ZT<ZPT> &T, const Grid &g, double (&F)[ZPT
         if (!T.has[s] || T.z[s] == 0 || T.z[s] == T.nz) continue;
         double pm, pp;
         phi_only(g, T.K, T.q00, T.z[s], pm, pp);
+#pragma unroll
         for (int m = 0; m < 4; ++m) F[s][m] = F[s][m] + sign * (lo[m] * pm + hi[m] * pp);
     }
 }
@@ -479,12 +492,14 @@ __device__ inline void zmid(const ZT<ZPT> &T, const ZLds &L, const double (&F)[Z
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
         if (T.has[s])
+#pragma unroll
             for (int m = 0; m < 4; ++m) cb[m * nz1 + T.z[s]] = F[s][m];
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < ZPT; ++s) {
         const int z = T.z[s];
         if (!T.has[s]) continue;
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             if (!T.store[m]) continue;
             const double v = (z > 0) ? 0.5 * (cb[m * nz1 + z - 1] + cb[m * nz1 + z]) : 0.0;
@@ -501,6 +516,7 @@ __device__ inline void mid_store(const ZT<ZPT> &T, const double *cb, double *__r
     for (int s = 0; s < ZPT; ++s) {
         const int z = T.z[s];
         if (!T.has[s]) continue;
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             if (!T.store[m]) continue;
             dst[T.col[m] + z] = (z > 0) ? 0.5 * (cb[m * nz1 + z - 1] + cb[m * nz1 + z]) : 0.0;
@@ -526,6 +542,7 @@ template <int ZPT>
 __device__ inline void zload(const ZT<ZPT> &T, const double *__restrict__ src, double (&F)[ZPT][4]) {
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
+#pragma unroll
         for (int m = 0; m < 4; ++m) F[s][m] = T.has[s] ? src[T.col[m] + T.z[s]] : 0.0;
 }
 template <int ZPT>
@@ -533,6 +550,7 @@ __device__ inline void zstore(const ZT<ZPT> &T, double *__restrict__ dst, const 
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
         if (T.has[s])
+#pragma unroll
             for (int m = 0; m < 4; ++m)
                 if (T.store[m]) dst[T.col[m] + T.z[s]] = F[s][m];
 }
@@ -561,6 +579,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     // D = B_x - A_y (inversion.f90:41-45)
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
+#pragma unroll
         for (int m = 0; m < 4; ++m) D[s][m] = T.fx[m] * Et[s][m ^ 1] - T.fy[m] * Xi[s][m ^ 2];
     // C = combine(zeta); E = decompose(central_diffz(C)) (46-48)
     {
@@ -578,6 +597,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     double Xp[ZPT][4], Ep[ZPT][4];
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             if (T.q00) {
                 Xp[s][m] = Xi[s][m];
@@ -596,6 +616,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     double Ds[ZPT][4];
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
+#pragma unroll
         for (int m = 0; m < 4; ++m) Ds[s][m] = T.fy[m] * Xp[s][m ^ 2] - T.fx[m] * Ep[s][m ^ 1];
     // lid values of ds (boundary part of w) and of the projected A, B (mean flow
     // of the (0,0) column and the combine below)
@@ -613,6 +634,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
         const Hyp h = hyp_all(g, T.K, T.k2l2i, T.q00, T.has[s] ? z : 0);
         const double rkz = a.rkz_sc * (double)z;
         const double green = -1.0 / (T.k2l2 + rkz * rkz);
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             Bs[s][m] = d0[m] * h.thetam + dn[m] * h.thetap;
             E[s][m] = d0[m] * h.dthetam + dn[m] * h.dthetap;     // es
@@ -636,6 +658,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
 #pragma unroll
     for (int s = 0; s < ZPT; ++s) {
         const int z = T.z[s];
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             if (z == 0 || z == nz)
                 Ds[s][m] = 0.0;
@@ -668,6 +691,7 @@ __global__ void __launch_bounds__(512) k_zinv(ZInvArgs a) {
     double U[ZPT][4], V[ZPT][4];
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             if (T.q00) {
                 U[s][m] = ubar[s];
@@ -720,6 +744,7 @@ __device__ inline void rk_update(const ZT<ZPT> &T, const RkArgs &rk, int mode, i
                 const int z = T.z[s];
                 double fz = f0;
                 if (z > 0 && z < T.nz && !T.q00) fz = tb.hou_li ? f0 * exp(tb.filz[z]) : f0 * tb.filz[z];
+#pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     if (!T.store[m]) continue;
                     const size_t i = T.col[m] + z;
@@ -739,6 +764,7 @@ __device__ inline void rk_update(const ZT<ZPT> &T, const RkArgs &rk, int mode, i
     // impl-diff-rk4: vdiss = dfac * vhdis, dfac = vorch*dt/2 (impl_rk4.f90:37-53);
     // epq = exp(vdiss) filt0, emq = 1/exp(vdiss) (87-89)
     const double dt2 = 0.5 * dt, dt3 = dt / 3.0, dt6 = dt / 6.0;
+#pragma unroll
     for (int m = 0; m < 4; ++m) {
         if (!T.store[m]) continue;
         const double D = rk.vdiss ? rk.vdiss[T.colidx[m]] : sc[SC_DFAC] * vhdis;
@@ -805,6 +831,7 @@ __global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
         if (!T.has[s] || T.z[s] == 0 || T.z[s] == T.nz) continue;
         double pm, pp;
         phi_only(g, T.K, T.q00, T.z[s], pm, pp);
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             R[s][m] = R[s][m] - (lo3[0][m] * pm + hi3[0][m] * pp);
             Q[s][m] = Q[s][m] - (lo3[1][m] * pm + hi3[1][m] * pp);
@@ -820,6 +847,7 @@ __global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
     double S[3][ZPT][4];
 #pragma unroll
     for (int s = 0; s < ZPT; ++s)
+#pragma unroll
         for (int m = 0; m < 4; ++m) {
             S[0][s][m] = T.fy[m] * R[s][m ^ 2] - FQ[s][m];
             S[1][s][m] = FP[s][m] - T.fx[m] * R[s][m ^ 1];
@@ -830,6 +858,7 @@ __global__ void __launch_bounds__(512) k_ztend(ZTendArgs a) {
 #pragma unroll
         for (int s = 0; s < ZPT; ++s)
             if (T.has[s])
+#pragma unroll
                 for (int m = 0; m < 4; ++m)
                     for (int c = 0; c < 3; ++c) S[c][s][m] += a.extra[c][T.col[m] + T.z[s]];
     }
@@ -866,6 +895,7 @@ __global__ void __launch_bounds__(512) k_zop(ZOpArgs a) {
 #pragma unroll
             for (int s = 0; s < ZPT; ++s)
                 if (T.has[s] && T.z[s] == T.nz)
+#pragma unroll
                     for (int m = 0; m < 4; ++m) F[s][m] = 0.0;
             zstore(T, a.out, F);
             break;
@@ -896,6 +926,7 @@ __global__ void __launch_bounds__(512) k_zop(ZOpArgs a) {
                 const int z = T.z[s];
                 const double rkz = a.rkz_sc * (double)z;
                 const double green = z == 0 ? -T.k2l2i : -1.0 / (T.k2l2 + rkz * rkz);
+#pragma unroll
                 for (int m = 0; m < 4; ++m) F[s][m] = green * F[s][m];
             }
             zdct4<ZPT, NZ>(T, L, F, a.pz, a.twz, a.zscale);
