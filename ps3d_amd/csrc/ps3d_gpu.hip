@@ -1445,8 +1445,9 @@ static void spectrum_shells(const ps3d_handle *h, int *kmax, double *dk, double 
     const double nx = h->g.nx, ny = h->g.ny, nz = h->g.nz;
     *dk = (double)*kmax / std::sqrt((0.5 * nx) * (0.5 * nx) + (0.5 * ny) * (0.5 * ny) + nz * nz);
     *dki = 1.0 / *dk;
-    // shells the binning can reach: m <= int(kmax * dki)
-    *nbins = std::max(*kmax, (int)((double)*kmax * *dki)) + 1;
+    // shells the binning keeps: m <= int(kmax * dki) is all it can reach, and shells
+    // beyond kmax (reachable when dk < 1) fall outside genspec's spec(0:kmax)
+    *nbins = std::min(*kmax, (int)((double)*kmax * *dki)) + 1;
 }
 
 // genspec.f90:101-117: shell m scaled by the volume of its spherical shell over its
@@ -1523,10 +1524,11 @@ int ps3d_gpu_spectrum(ps3d_handle *h, const double *vel, int cap, int *nshell, d
         HIPCHK(launch_spectrum(a, h->stream));
     }
     HIPCHK(launch_spectrum_fin(h->spec_part, h->spec_cpart, a.nb, nbins, h->spec_out, h->spec_cnt, h->stream));
-    std::vector<double> raw(kmax + 1);
-    std::vector<long long> cnt(kmax + 1);
-    HIPCHK(hipMemcpyAsync(raw.data(), h->spec_out, sizeof(double) * (kmax + 1), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(cnt.data(), h->spec_cnt, sizeof(long long) * (kmax + 1), hipMemcpyDeviceToHost, h->stream));
+    // shells nbins..kmax (dk > 1: beyond the reach of every point) stay empty
+    std::vector<double> raw(kmax + 1, 0.0);
+    std::vector<long long> cnt(kmax + 1, 0);
+    HIPCHK(hipMemcpyAsync(raw.data(), h->spec_out, sizeof(double) * nbins, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(cnt.data(), h->spec_cnt, sizeof(long long) * nbins, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     // shells beyond kmax (reachable only when dk < 1) are not part of genspec's spectrum
     spectrum_scale(kmax, dkv, ke, raw.data(), cnt.data(), spec);
