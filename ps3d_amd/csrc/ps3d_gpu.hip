@@ -363,13 +363,8 @@ static int init_handle(ps3d_handle *h, const ps3d_params *p, bool alloc_fields, 
         // register variant: the two-waves-per-SIMD builds (no spills, the update's
         // operands loaded ahead) measure fastest at nz = 256 and 512
         int var = 2;
-        const bool asked = ev && ev[0] >= '1' && ev[0] <= '3';
-        if (asked) var = ev[0] - '0';
-        // nz = 64: the quad kernels measure faster on a single grid (0.61 against
-        // 0.99 ms per 64^3 step); the wave kernels there serve when asked for and as
-        // a candidate of the ensemble tuner
-        const bool dflt = nz >= 128;
-        h->zl.wave = (ps3d::zwave_supported(nz) && !off && (dflt || asked)) ? var : 0;
+        if (ev && ev[0] >= '1' && ev[0] <= '3') var = ev[0] - '0';
+        h->zl.wave = (ps3d::zwave_supported(nz) && !off) ? var : 0;
         const char *fz = std::getenv("PS3D_ZFUSE");
         h->zfuse = !(fz && fz[0] == '0');
     }

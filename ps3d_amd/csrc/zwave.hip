@@ -262,8 +262,8 @@ __device__ inline WTw<P> make_tw(const double2 *__restrict__ tw, int lane) {
     return WTw<P>{tw};
 }
 
-// Forward complex FFT of length 64 P.  In and out: lane l, register j holds
-// element l + 64 j.  ws: the wave's LDS slice (72 P double2).
+// Forward complex FFT of length 64 P (P = 8, 16).  In and out: lane l, register j
+// holds element l + 64 j.  ws: the wave's LDS slice (72 P double2).
 template <int P>
 __device__ inline void wfft(double2 (&x)[P], double2 *__restrict__ ws, const WTw<P> &tw, int lane) {
     constexpr int IPL = (8 * P + 63) / 64;
@@ -294,8 +294,6 @@ __device__ inline void wfft(double2 (&x)[P], double2 *__restrict__ ws, const WTw
 #pragma unroll
     for (int k = 0; k < P; ++k) ws[k * 64 + lane] = x[k];
     wave_sync();
-    // (for P < 8 the upper lanes repeat the items of the lower ones: same values
-    // to the same addresses, and no divergent branch)
     double2 u[IPL][8];
 #pragma unroll
     for (int r = 0; r < IPL; ++r) {
@@ -329,31 +327,92 @@ __device__ inline void wfft(double2 (&x)[P], double2 *__restrict__ ws, const WTw
     if constexpr (P == 8) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) x[j] = u[0][j];
-    } else if constexpr (P == 16) {
+    } else {
+        static_assert(P == 16, "8 x 8 lane stages for P = 8, 16 (wfft4 takes P = 2, 4)");
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             x[2 * j] = u[0][j];
             x[2 * j + 1] = u[1][j];
         }
-    } else if constexpr (P == 2) {
-        // item it = lane mod 16 holds X[it + 16 k2b] in u[0][k2b]; element lane + 64 j
-        // is X[(lane mod 16) + 16 ((lane >> 4) + 4 j)]: this lane's own item
-        const int h = lane >> 4;
+    }
+}
+
+// Forward complex FFT of length 64 P for P = 2, 4, same layout as wfft: the 64-point
+// DFTs across the lanes as 4 x 4 x 4, three radix-4 stages through the wave's LDS
+// slice.  The 8 x 8 form has 8 P items per stage, so at P = 4 half the lanes and at
+// P = 2 three quarters repeat the others' work; here there are 16 P items.
+//   stage 1: item (k1, m), m < 16: DFT over l3 of row k1's elements m + 16 l3 -> k2a,
+//            twiddle w_64^(m k2a);
+//   stage 2: item (k1, k2a, l1): DFT over l2 of stage 1's (k1, k2a, l1 + 4 l2) -> qa,
+//            twiddle w_16^(l1 qa);
+//   stage 3: item k1 + P k2a + 4 P qa: DFT over l1 -> X[item + 16 P qb].
+template <int P>
+__device__ inline void wfft4(double2 (&x)[P], double2 *__restrict__ ws, const WTw<P> &tw, int lane) {
+    static_assert(P == 2 || P == 4, "radix-4 lane stages for P = 2, 4");
+    constexpr int NIT = 16 * P;
+    const double2 *tg = tw.g;
+    asm volatile("" : "+s"(tg));
+    double2 ta[P];
+    ta[1] = tg[lane];
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
-            x[j] = h == 0 ? u[0][4 * j] : h == 1 ? u[0][4 * j + 1] : h == 2 ? u[0][4 * j + 2] : u[0][4 * j + 3];
+    for (int k = 2; k < P; ++k) ta[k] = cmul(ta[k / 2], ta[k - k / 2]);
+    // length-P DFT over the registers, then the twiddle w_N^(lane k1)
+    bfly<P>(x, -1);
+#pragma unroll
+    for (int k = 1; k < P; ++k) x[k] = cmul(x[k], ta[k]);
+#pragma unroll
+    for (int k = 0; k < P; ++k) ws[k * 64 + lane] = x[k];
+    wave_sync();
+    // (at P = 2 the upper lanes repeat the items of the lower ones: same values to
+    // the same addresses)
+    const int it = lane % NIT;
+    double2 u[4];
+    {
+        const int k1 = it >> 4, m = it & 15;
+#pragma unroll
+        for (int l3 = 0; l3 < 4; ++l3) u[l3] = ws[k1 * 64 + m + 16 * l3];
+        const double2 w1 = tg[P * m];
+        const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        wave_sync();
+        bfly<4>(u, -1);
+        u[1] = cmul(u[1], w1);
+        u[2] = cmul(u[2], w2);
+        u[3] = cmul(u[3], w3);
+#pragma unroll
+        for (int k2a = 0; k2a < 4; ++k2a) ws[k1 * 64 + k2a * 16 + m] = u[k2a];
+    }
+    wave_sync();
+    {
+        const int k1 = it >> 4, k2a = (it >> 2) & 3, l1 = it & 3;
+#pragma unroll
+        for (int l2 = 0; l2 < 4; ++l2) u[l2] = ws[k1 * 64 + k2a * 16 + l1 + 4 * l2];
+        const double2 w1 = tg[4 * P * l1];
+        const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        wave_sync();
+        bfly<4>(u, -1);
+        u[1] = cmul(u[1], w1);
+        u[2] = cmul(u[2], w2);
+        u[3] = cmul(u[3], w3);
+#pragma unroll
+        for (int qa = 0; qa < 4; ++qa) ws[((qa * 4 + k2a) * 4 + l1) * P + k1] = u[qa];
+    }
+    wave_sync();
+    {
+        const int k1 = it % P, k2a = (it / P) & 3, qa = it / (4 * P);
+#pragma unroll
+        for (int l1 = 0; l1 < 4; ++l1) u[l1] = ws[((qa * 4 + k2a) * 4 + l1) * P + k1];
+        bfly<4>(u, -1);
+    }
+    wave_sync();   // the slice is free for the caller again
+    if constexpr (P == 4) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[j] = u[j];
     } else {
-        static_assert(P == 4, "wave FFT lengths: 128, 256, 512, 1024");
-        // lanes < 32 hold X[lane + 32 k2b]; element lane + 64 j with lane >= 32 is
-        // X[(lane - 32) + 32 (2 j + 1)]
-        const int src = lane & 31;
-        const double2 t0 = shfl2(u[0][1], src), t1 = shfl2(u[0][3], src);
-        const double2 t2 = shfl2(u[0][5], src), t3 = shfl2(u[0][7], src);
-        const bool lo = lane < 32;
-        x[0] = lo ? u[0][0] : t0;
-        x[1] = lo ? u[0][2] : t1;
-        x[2] = lo ? u[0][4] : t2;
-        x[3] = lo ? u[0][6] : t3;
+        // item lane mod 32 holds X[item + 32 qb]; element lane + 64 j is
+        // X[(lane mod 32) + 32 ((lane >> 5) + 2 j)]
+        const bool hi = lane >= 32;
+        x[0] = hi ? u[1] : u[0];
+        x[1] = hi ? u[3] : u[2];
     }
 }
 
@@ -401,7 +460,10 @@ __device__ inline void wxform(const WC &c, WF<NZ> &F0, WF<NZ> &F1, double2 *ws, 
     double2 x[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) x[j] = make_double2(a[j], b[j]);
-    wfft<P>(x, ws, tw, c.lane);
+    if constexpr (P <= 4)
+        wfft4<P>(x, ws, tw, c.lane);
+    else
+        wfft<P>(x, ws, tw, c.lane);
     const int l = c.lane;
     if constexpr (K0 == K1 || K1 == WK_NONE) {
 #pragma unroll

@@ -321,7 +321,7 @@ def test_grid_with_prime_factor_7_is_refused(G):
         G.Ps3dGpu(G.make_params(**dict(box(28), nz=16), **config1()))
 
 
-@pytest.mark.parametrize("nz", [64, 256])
+@pytest.mark.parametrize("nz", [64, 128, 256])
 @pytest.mark.parametrize("variant", ["1", "2", "3"])
 def test_wave_and_quad_z_kernels_agree(G, monkeypatch, variant, nz):
     """The wave-per-column (both register budgets) and quad-per-thread z kernels
