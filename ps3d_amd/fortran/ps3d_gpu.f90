@@ -21,6 +21,8 @@ module ps3d_gpu
     integer, parameter :: PS3D_NDIAG = 19
     integer, parameter :: PS3D_NSTAT = 40
     integer, parameter :: PS3D_NC_STATS_LEN = PS3D_NSTAT + 4
+    ! ps3d_gpu_moments: ke, helicity, cross-helicity mean and largest |u x w|
+    integer, parameter :: PS3D_NMOMENT = 4
 
     type, bind(C) :: ps3d_params
         integer(c_int) :: nx, ny, nz
@@ -86,6 +88,23 @@ module ps3d_gpu
             type(c_ptr), value :: h
             type(c_ptr), value :: diag
             real(c_double), intent(out) :: stats(*)
+        end function
+        integer(c_int) function ps3d_gpu_moments(h, out) bind(C)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(out) :: out(*)
+        end function
+        ! genspec's spectrum: vel = c_null_ptr for the state's velocity; spec = c_null_ptr
+        ! (with cap 0) only sets nshell and dk
+        integer(c_int) function ps3d_gpu_spectrum(h, vel, cap, nshell, dk, spec, num) bind(C)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            type(c_ptr), value :: vel
+            integer(c_int), value :: cap
+            integer(c_int), intent(out) :: nshell
+            real(c_double), intent(out) :: dk
+            type(c_ptr), value :: spec
+            type(c_ptr), value :: num
         end function
         integer(c_int) function ps3d_gpu_get(h, which, f) bind(C)
             import :: c_int, c_ptr, c_double
