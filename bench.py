@@ -43,6 +43,8 @@ def parse():
                          "E(k) ~ k^4 exp(-2 (k/4)^2) made on 128^3 and zero-padded to n^3 on the GPU")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-instance ensemble measurement")
     ap.add_argument("--no-closure", action="store_true", help="skip the closure-64^3 vs plain-256^3 measurement")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other grid sizes (32^3, 128^3, 512^3, 1024^3)")
     ap.add_argument("--closure-seed", type=int, default=9001, help="held-out seed of the closure measurement")
     ap.add_argument("--closure-steps", type=int, default=1000, help="fine 256^3 steps of the closure measurement")
     ap.add_argument("--ensemble-steps", type=int, default=100)
@@ -179,6 +181,43 @@ def closure_bench(G, seed, fine_steps, timing_steps):
         "enstrophy_error": {k: ev[k]["enstrophy_error"] for k, _, _ in cfg},
         "target": "closure 64^3 >= 20x faster than plain 256^3 at <= 5% spectrum error",
     }
+
+
+# The other configurations of the workload, measured after the headline one:
+# (label, n, initial condition, fixed dt (0: adaptive), timed steps)
+CONFIG_RUNS = [("config 1: 32^3 Beltrami, fixed dt 0.01", 32, "beltrami", 0.01, 100),
+               ("config 2: 128^3 Beltrami, adaptive dt", 128, "beltrami", 0.0, 100),
+               ("config 3: 512^3 random-phase isotropic, adaptive dt", 512, "random", 0.0, 10),
+               ("config 3: 1024^3 random-phase isotropic, adaptive dt", 1024, "random", 0.0, 3)]
+
+
+def configs_bench(G, ics):
+    """Steps/s, grid-point updates/s and the step roofline of every other grid size
+    of the workload (one grid at a time, resident in HBM, graph-replayed steps)."""
+    out = []
+    for label, n, ic, dt, steps in CONFIG_RUNS:
+        p = workload_params(G, n)
+        p.dt_fixed = dt
+        vor, coarse = initial_vorticity(G, ics, n, ic)
+        d = G.Ps3dGpu(p)
+        if coarse is None:
+            d.set_vorticity(vor)
+        else:
+            G.resample(d, coarse)
+            coarse.close()
+        d.advance_n(2)
+        d.sync()
+        t0 = time.perf_counter()
+        d.advance_n(steps)
+        el = time.perf_counter() - t0
+        d.close()
+        npts = n * n * (n + 1)
+        sps = steps / el
+        b_step = B_STEP_VOLUMES * 8.0 * npts
+        out.append({"config": label, "grid": [n, n, n], "steps": steps, "steps_per_s": sps,
+                    "ms_per_step": 1e3 * el / steps, "grid_point_updates_per_s": npts * sps,
+                    "step_roofline_frac": b_step * sps / PEAK_HBM})
+    return out
 
 
 def initial_vorticity(G, ics, n, kind):
@@ -353,6 +392,11 @@ def main():
             out["ensemble"] = ensemble_bench(G, a.ensemble_steps, a.separate_steps)
         except Exception as exc:       # reported, never silently dropped
             out["ensemble"] = {"error": repr(exc)}
+    if not a.no_configs:
+        try:
+            out["configs"] = configs_bench(G, ics)
+        except Exception as exc:       # reported, never silently dropped
+            out["configs"] = {"error": repr(exc)}
     if not a.no_closure:
         try:
             out["closure"] = closure_bench(G, a.closure_seed, a.closure_steps, 20)
