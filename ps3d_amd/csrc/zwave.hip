@@ -249,17 +249,37 @@ __device__ inline void wharm(const WC &c, const WPhi<NZ> &h, WF<NZ> &F, double l
         if (c.lane + 64 * s > 0) F.v[s] = F.v[s] + sign * (lo * h.pm[s] + hi * h.pp[s]);
 }
 
-// Twiddles of the wave FFT of length N = 64 P, read from the global table
-// tw[m] = exp(-2 pi i m / N) at each transform (L1/L2 resident): w_N^(lane k1) for
-// the register pass, w_64^(l1 k2a) for the lane pass.  Held only during the FFT.
+// Twiddles of the wave FFT of length N = 64 P.  The lane's roots are read from the
+// global table tw[m] = exp(-2 pi i m / N) once per kernel (those loads overlap the
+// column loads); each transform forms the powers it needs from them by binary
+// splitting.  Reading the table inside every transform made each transform's first
+// use wait for all of the wave's global accesses in flight (the vector memory
+// counter is in order), stores of the previous phase included.
+//   r1 = w_N^lane (register pass);
+//   P = 8, 16: a = w_64^(lane mod 8) (8 x 8 lane stage);
+//   P = 2, 4:  a = w_64^(lane mod 16), b = w_16^(lane mod 4) (radix-4 lane stages).
 template <int P>
 struct WTw {
-    const double2 *g;
+    double2 r1, a, b;
 };
 template <int P>
 __device__ inline WTw<P> make_tw(const double2 *__restrict__ tw, int lane) {
-    (void)lane;
-    return WTw<P>{tw};
+    WTw<P> t;
+    t.r1 = tw[lane];
+    if constexpr (P >= 8) {
+        t.a = tw[P * (lane & 7)];
+        t.b = t.a;
+    } else {
+        t.a = tw[P * (lane & 15)];
+        t.b = tw[4 * P * (lane & 3)];
+    }
+    return t;
+}
+// An opaque copy of a root: keeps the compiler from sharing the powers between
+// transforms (they would stay live across the kernel).
+__device__ inline double2 wroot(double2 r) {
+    asm volatile("" : "+v"(r.x), "+v"(r.y));
+    return r;
 }
 
 // Forward complex FFT of length 64 P (P = 8, 16).  In and out: lane l, register j
@@ -268,21 +288,17 @@ template <int P>
 __device__ inline void wfft(double2 (&x)[P], double2 *__restrict__ ws, const WTw<P> &tw, int lane) {
     constexpr int IPL = (8 * P + 63) / 64;
     constexpr int NIT = 8 * P;
-    // powers of one table entry by binary splitting (product depth <= log2 k;
+    // powers of the lane's roots by binary splitting (product depth <= log2 k;
     // measured: reading every power from the table instead made the fused
-    // tendency + vor2vel kernel 1.7x slower).
-    // The opaque copy of the table pointer keeps the compiler from sharing the
-    // twiddles between transforms (they would stay live across the kernel).
-    const double2 *tg = tw.g;
-    asm volatile("" : "+s"(tg));
+    // tendency + vor2vel kernel 1.7x slower).  Item lane + 64 r of the lane stage
+    // has l1 = lane mod 8 for every r (8 P is a multiple of 64).
     double2 ta[P], tb[IPL][8];
-    ta[1] = tg[lane];
+    ta[1] = wroot(tw.r1);
 #pragma unroll
     for (int k = 2; k < P; ++k) ta[k] = cmul(ta[k / 2], ta[k - k / 2]);
 #pragma unroll
     for (int r = 0; r < IPL; ++r) {
-        const int l1 = ((lane + 64 * r) % NIT) & 7;
-        tb[r][1] = tg[P * l1];
+        tb[r][1] = wroot(tw.a);
 #pragma unroll
         for (int k = 2; k < 8; ++k) tb[r][k] = cmul(tb[r][k / 2], tb[r][k - k / 2]);
     }
@@ -350,10 +366,8 @@ template <int P>
 __device__ inline void wfft4(double2 (&x)[P], double2 *__restrict__ ws, const WTw<P> &tw, int lane) {
     static_assert(P == 2 || P == 4, "radix-4 lane stages for P = 2, 4");
     constexpr int NIT = 16 * P;
-    const double2 *tg = tw.g;
-    asm volatile("" : "+s"(tg));
     double2 ta[P];
-    ta[1] = tg[lane];
+    ta[1] = wroot(tw.r1);
 #pragma unroll
     for (int k = 2; k < P; ++k) ta[k] = cmul(ta[k / 2], ta[k - k / 2]);
     // length-P DFT over the registers, then the twiddle w_N^(lane k1)
@@ -371,7 +385,7 @@ __device__ inline void wfft4(double2 (&x)[P], double2 *__restrict__ ws, const WT
         const int k1 = it >> 4, m = it & 15;
 #pragma unroll
         for (int l3 = 0; l3 < 4; ++l3) u[l3] = ws[k1 * 64 + m + 16 * l3];
-        const double2 w1 = tg[P * m];
+        const double2 w1 = wroot(tw.a);   // w_64^m, m = lane mod 16
         const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
         wave_sync();
         bfly<4>(u, -1);
@@ -386,7 +400,7 @@ __device__ inline void wfft4(double2 (&x)[P], double2 *__restrict__ ws, const WT
         const int k1 = it >> 4, k2a = (it >> 2) & 3, l1 = it & 3;
 #pragma unroll
         for (int l2 = 0; l2 < 4; ++l2) u[l2] = ws[k1 * 64 + k2a * 16 + l1 + 4 * l2];
-        const double2 w1 = tg[4 * P * l1];
+        const double2 w1 = wroot(tw.b);   // w_16^l1, l1 = lane mod 4
         const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
         wave_sync();
         bfly<4>(u, -1);
@@ -544,9 +558,54 @@ __device__ __attribute__((always_inline)) inline void wrk_load(const WC &c, cons
     if (two) wload(c, rk.qdf[cc], o.b);
 }
 
+// The column's RK coefficients.  wrk_setup reads their inputs (the step's scalars,
+// the filter and dissipation of the column: wave-uniform) where they are used;
+// the standalone tendency kernel reads them at its top instead (wrk_inputs), so that
+// those loads overlap the column loads rather than wait, in the middle of the pass,
+// behind the operands in flight (measured: 0.396 -> 0.380 ms per step at 256^3; in
+// the fused kernel the early reads measured slower, 2.37 -> 2.42 ms).
 struct WRk {
     double dt, f0, vd, em, epf;
 };
+struct WRkIn {
+    double dt, visc, dfac, fx, fy, vdiss;
+};
+__device__ inline WRkIn wrk_inputs(const WC &c, const RkArgs &rk, const ZTabs &tb) {
+    WRkIn in;
+    const double *sc = rk.scal + (size_t)blockIdx.y * SC_NSCAL;
+    in.dt = sc[SC_DT];
+    in.visc = sc[SC_VISC];
+    in.dfac = sc[SC_DFAC];
+    in.fx = tb.filx[c.kx];
+    in.fy = tb.fily[c.ky];
+    in.vdiss = rk.vdiss ? rk.vdiss[c.colidx] : 0.0;
+    return in;
+}
+template <int NZ>
+__device__ inline WRk wrk_setup_from(const WC &c, const RkArgs &rk, int mode, const ZTabs &tb, const WRkIn &in) {
+    WRk r;
+    r.dt = in.dt;
+    if (c.q00)
+        r.f0 = 1.0;
+    else if (tb.hou_li)
+        r.f0 = exp(in.fx + in.fy);
+    else
+        r.f0 = in.fx * in.fy;
+    double kp = 1.0;
+    for (int i = 0; i < tb.nnu; ++i) kp *= c.k2l2;
+    const double vhdis = in.visc * kp;
+    if (mode == ZT_CN2) {
+        r.vd = 1.0 / (1.0 + in.dfac * vhdis);
+        r.em = r.epf = 0.0;
+    } else {
+        const double D = rk.vdiss ? in.vdiss : in.dfac * vhdis;
+        const double ep = exp(D);
+        r.em = 1.0 / ep;
+        r.epf = ep * r.f0;
+        r.vd = 0.0;
+    }
+    return r;
+}
 template <int NZ>
 __device__ inline WRk wrk_setup(const WC &c, const RkArgs &rk, int mode, const ZTabs &tb) {
     WRk r;
@@ -659,17 +718,18 @@ __device__ inline void wzero(WF<NZ> &F) {
 // wave per member.  Same inputs and outputs as the quad-per-thread k_zinv.
 // ---------------------------------------------------------------------------
 // OCC > 1 asks the compiler for OCC waves per SIMD (fewer registers, possibly
-// spills); OCC = 1 leaves it free.  Both variants are built; the launch shape
-// (and the ensemble tuner) choose.
+// spills); OCC = 1 leaves it free, except at nz <= 128, where it asks for four
+// (12-36 bytes per lane of spills, measured faster than three waves without).  Both
+// variants are built; the launch shape (and the ensemble tuner) choose.
 // The body: svor from memory, or (reg != nullptr) the three components already in
 // this wave's registers (the fused tendency + next-stage pass below).
 template <int NZ>
 __device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &a, const WC &c, double2 *wsm,
-                                                                const WF<NZ> *reg, const WSeed &sd) {
-    constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
+                                                                const WF<NZ> *reg, const WSeed &sd,
+                                                                const WTw<WDim<NZ>::P> &tw) {
+    constexpr int S = WDim<NZ>::S;
     const Grid g = a.g;
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
-    const WTw<P> tw = make_tw<P>(a.twz, c.lane);
     const double sc = a.zscale;
     const int l = c.lane;
     const int m1 = c.m ^ 1, m2 = c.m ^ 2;
@@ -872,11 +932,11 @@ __device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &
 }
 
 template <int NZ, int OCC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_zinv_w(ZInvArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NZ <= 128 && OCC == 1) ? 4 : OCC, 8))) k_zinv_w(ZInvArgs a) {
     extern __shared__ double2 wsm[];
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
-    zinv_body<NZ>(a, c, wsm, nullptr, make_wseed(c, a.g));
+    zinv_body<NZ>(a, c, wsm, nullptr, make_wseed(c, a.g), make_tw<WDim<NZ>::P>(a.twz, c.lane));
 }
 
 // ---------------------------------------------------------------------------
@@ -887,11 +947,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
 // next stage's vor2vel pass instead of going to memory.
 template <int NZ, bool KEEP>
 __device__ __attribute__((always_inline)) inline void ztend_body(const ZTendArgs &a, const WC &c, double2 *wsm,
-                                                                 WF<NZ> *keep, const WSeed &sd) {
-    constexpr int S = WDim<NZ>::S, P = WDim<NZ>::P;
+                                                                 WF<NZ> *keep, const WSeed &sd,
+                                                                 const WTw<WDim<NZ>::P> &tw, const WRkIn *rin) {
+    constexpr int S = WDim<NZ>::S;
     const Grid g = a.g;
     double2 *ws = wsm + (size_t)c.m * WDim<NZ>::WS;
-    const WTw<P> tw = make_tw<P>(a.twz, c.lane);
     const double sc = a.zscale;
 
     WF<NZ> R, Q, Pp, FQ, FP;
@@ -959,7 +1019,7 @@ __device__ __attribute__((always_inline)) inline void ztend_body(const ZTendArgs
         for (int cc = 0; cc < 3; ++cc) wstore(c, a.svorts[cc], St[cc]);
         return;
     }
-    const WRk rr = wrk_setup<NZ>(c, a.rk, a.mode, a.tb);
+    const WRk rr = rin ? wrk_setup_from<NZ>(c, a.rk, a.mode, a.tb, *rin) : wrk_setup<NZ>(c, a.rk, a.mode, a.tb);
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) {
         WOps<NZ> nxt;
@@ -970,11 +1030,12 @@ __device__ __attribute__((always_inline)) inline void ztend_body(const ZTendArgs
 }
 
 template <int NZ, int OCC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8))) k_ztend_w(ZTendArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NZ <= 128 && OCC == 1) ? 4 : OCC, 8))) k_ztend_w(ZTendArgs a) {
     extern __shared__ double2 wsm[];
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
-    ztend_body<NZ, false>(a, c, wsm, nullptr, make_wseed(c, a.g));
+    const WRkIn rin = wrk_inputs(c, a.rk, a.tb);
+    ztend_body<NZ, false>(a, c, wsm, nullptr, make_wseed(c, a.g), make_tw<WDim<NZ>::P>(a.twz, c.lane), &rin);
 }
 
 // RK4 stage s (1..3) of the tendency pass and the vor2vel z pass of stage s+1 in
@@ -983,18 +1044,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8
 // At nz = 512 one barrier separates the two partner exchanges' uses of the block's
 // LDS.
 template <int NZ, int OCC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NZ <= 128 && OCC == 1) ? 4 : OCC, 8)))
 k_ztend_zinv_w(ZTendArgs a, ZInvArgs b) {
     extern __shared__ double2 wsm[];
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
     WF<NZ> q[3];
     const WSeed sd = make_wseed(c, a.g);
-    ztend_body<NZ, true>(a, c, wsm, q, sd);
+    const WTw<WDim<NZ>::P> tw = make_tw<WDim<NZ>::P>(a.twz, c.lane);   // one set of roots for both passes
+    ztend_body<NZ, true>(a, c, wsm, q, sd, tw, nullptr);
     // (with the second region the vor2vel pass's first exchange does not touch the
     // slices the tendency pass's exchange used)
     if constexpr (!WDim<NZ>::DB) __syncthreads();
-    zinv_body<NZ>(b, c, wsm, q, sd);
+    zinv_body<NZ>(b, c, wsm, q, sd, tw);
 }
 
 // Single z operators on one volume (kernel checks of the wave path).
