@@ -725,7 +725,8 @@ __device__ inline void wzero(WF<NZ> &F) {
 // this wave's registers (the fused tendency + next-stage pass below).
 template <int NZ>
 __device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &a, const WC &c, double2 *wsm,
-                                                                const WF<NZ> *reg, const WSeed &sd,
+                                                                const WF<NZ> *reg0, const WF<NZ> *reg1,
+                                                                const WF<NZ> *reg2, const WSeed &sd,
                                                                 const WTw<WDim<NZ>::P> &tw) {
     constexpr int S = WDim<NZ>::S;
     const Grid g = a.g;
@@ -735,10 +736,10 @@ __device__ __attribute__((always_inline)) inline void zinv_body(const ZInvArgs &
     const int m1 = c.m ^ 1, m2 = c.m ^ 2;
 
     WF<NZ> Xi, Et, C, D, E;
-    if (reg) {
-        Xi = reg[0];
-        Et = reg[1];
-        C = reg[2];
+    if (reg0) {
+        Xi = *reg0;
+        Et = *reg1;
+        C = *reg2;
     } else {
         wload(c, a.svor[0], Xi);
         wload(c, a.svor[1], Et);
@@ -936,7 +937,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NZ <=
     extern __shared__ double2 wsm[];
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
-    zinv_body<NZ>(a, c, wsm, nullptr, make_wseed(c, a.g), make_tw<WDim<NZ>::P>(a.twz, c.lane));
+    zinv_body<NZ>(a, c, wsm, nullptr, nullptr, nullptr, make_wseed(c, a.g), make_tw<WDim<NZ>::P>(a.twz, c.lane));
 }
 
 // ---------------------------------------------------------------------------
@@ -947,7 +948,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NZ <=
 // next stage's vor2vel pass instead of going to memory.
 template <int NZ, bool KEEP>
 __device__ __attribute__((always_inline)) inline void ztend_body(const ZTendArgs &a, const WC &c, double2 *wsm,
-                                                                 WF<NZ> *keep, const WSeed &sd,
+                                                                 WF<NZ> *keep0, WF<NZ> *keep1, WF<NZ> *keep2,
+                                                                 const WSeed &sd,
                                                                  const WTw<WDim<NZ>::P> &tw, const WRkIn *rin) {
     constexpr int S = WDim<NZ>::S;
     const Grid g = a.g;
@@ -1020,12 +1022,28 @@ __device__ __attribute__((always_inline)) inline void ztend_body(const ZTendArgs
         return;
     }
     const WRk rr = rin ? wrk_setup_from<NZ>(c, a.rk, a.mode, a.tb, *rin) : wrk_setup<NZ>(c, a.rk, a.mode, a.tb);
+    if constexpr (NZ >= 512) {
+        // nz = 512: the three components written out in sequence, the kept results in
+        // three separate variables (as one array indexed in the loop below they were
+        // kept in scratch memory, 224 bytes per lane): fused kernel 30.6 -> 28.2-29.8
+        // ms per 512^3 step (two builds with identical code for it measured both ends:
+        // its 96 KB of code exceed the instruction cache, and where the library places
+        // it matters).  At nz <= 256 the loop and the array measured as fast or faster.
+        WOps<NZ> o1, o2;
+        wrk_load<NZ, KEEP>(c, a.rk, a.mode, a.stage, 1, o1);
+        wrk_apply<NZ, KEEP>(c, a.rk, a.mode, a.stage, 0, St[0], ops, rr, a.tb, KEEP ? keep0 : nullptr);
+        wrk_load<NZ, KEEP>(c, a.rk, a.mode, a.stage, 2, o2);
+        wrk_apply<NZ, KEEP>(c, a.rk, a.mode, a.stage, 1, St[1], o1, rr, a.tb, KEEP ? keep1 : nullptr);
+        wrk_apply<NZ, KEEP>(c, a.rk, a.mode, a.stage, 2, St[2], o2, rr, a.tb, KEEP ? keep2 : nullptr);
+    } else {
 #pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-        WOps<NZ> nxt;
-        if (cc < 2) wrk_load<NZ, KEEP>(c, a.rk, a.mode, a.stage, cc + 1, nxt);
-        wrk_apply<NZ, KEEP>(c, a.rk, a.mode, a.stage, cc, St[cc], ops, rr, a.tb, KEEP ? keep + cc : nullptr);
-        if (cc < 2) ops = nxt;
+        for (int cc = 0; cc < 3; ++cc) {
+            WOps<NZ> nxt;
+            if (cc < 2) wrk_load<NZ, KEEP>(c, a.rk, a.mode, a.stage, cc + 1, nxt);
+            // (below nz = 512 the kept results are one array: keep1 = keep0 + 1, ...)
+            wrk_apply<NZ, KEEP>(c, a.rk, a.mode, a.stage, cc, St[cc], ops, rr, a.tb, KEEP ? keep0 + cc : nullptr);
+            if (cc < 2) ops = nxt;
+        }
     }
 }
 
@@ -1035,7 +1053,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((NZ <=
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
     const WRkIn rin = wrk_inputs(c, a.rk, a.tb);
-    ztend_body<NZ, false>(a, c, wsm, nullptr, make_wseed(c, a.g), make_tw<WDim<NZ>::P>(a.twz, c.lane), &rin);
+    ztend_body<NZ, false>(a, c, wsm, nullptr, nullptr, nullptr, make_wseed(c, a.g), make_tw<WDim<NZ>::P>(a.twz, c.lane),
+                          &rin);
 }
 
 // RK4 stage s (1..3) of the tendency pass and the vor2vel z pass of stage s+1 in
@@ -1049,14 +1068,22 @@ k_ztend_zinv_w(ZTendArgs a, ZInvArgs b) {
     extern __shared__ double2 wsm[];
     if (gated_off(a.g.gate)) return;
     const WC c = make_wc(a.g, a.tb, (long)blockIdx.y * a.bt.vol);
-    WF<NZ> q[3];
     const WSeed sd = make_wseed(c, a.g);
     const WTw<WDim<NZ>::P> tw = make_tw<WDim<NZ>::P>(a.twz, c.lane);   // one set of roots for both passes
-    ztend_body<NZ, true>(a, c, wsm, q, sd, tw, nullptr);
-    // (with the second region the vor2vel pass's first exchange does not touch the
-    // slices the tendency pass's exchange used)
-    if constexpr (!WDim<NZ>::DB) __syncthreads();
-    zinv_body<NZ>(b, c, wsm, q, sd, tw);
+    // the stage results: three separate variables at nz = 512, one array below
+    if constexpr (NZ >= 512) {
+        WF<NZ> q0, q1, q2;
+        ztend_body<NZ, true>(a, c, wsm, &q0, &q1, &q2, sd, tw, nullptr);
+        __syncthreads();   // one LDS region: the exchanges of both passes use the slices
+        zinv_body<NZ>(b, c, wsm, &q0, &q1, &q2, sd, tw);
+    } else {
+        static_assert(WDim<NZ>::DB, "second exchange region below nz = 512");
+        // (with the second region the vor2vel pass's first exchange does not touch the
+        // slices the tendency pass's exchange used)
+        WF<NZ> q[3];
+        ztend_body<NZ, true>(a, c, wsm, q, q + 1, q + 2, sd, tw, nullptr);
+        zinv_body<NZ>(b, c, wsm, q, q + 1, q + 2, sd, tw);
+    }
 }
 
 // Single z operators on one volume (kernel checks of the wave path).
