@@ -455,19 +455,19 @@ unsigned blocks_for(long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 int free_cnn(ps3d_cnn *c) {
     if (!c) return 0;
-    if (c->own) hipStreamSynchronize(c->own);
+    if (c->own) (void)hipStreamSynchronize(c->own);
     void *ptrs[] = {c->w, c->gw, c->m, c->v, c->a0, c->y, c->gy, c->gpA, c->gpB, c->gx, c->gyb, c->gbA, c->gbB,
                     c->part, c->partb, c->lpart, c->loss, c->loss_acc, c->dx, c->dt, c->tgt};
-    for (void *p : ptrs) hipFree(p);
+    for (void *p : ptrs) (void)hipFree(p);
     for (int l = 0; l < NL; ++l) {
-        hipFree(c->wf[l]);
-        hipFree(c->wt[l]);
+        (void)hipFree(c->wf[l]);
+        (void)hipFree(c->wt[l]);
     }
     for (int l = 0; l < 3; ++l) {
-        hipFree(c->pre[l]);
-        hipFree(c->act[l]);
+        (void)hipFree(c->pre[l]);
+        (void)hipFree(c->act[l]);
     }
-    if (c->own) hipStreamDestroy(c->own);
+    if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
     return 0;
 }
@@ -755,8 +755,8 @@ int ps3d_cnn_train_step(ps3d_cnn *c, const float *x, const float *target, float 
 
 int ps3d_cnn_dataset_reserve(ps3d_cnn *c, int nsamples) {
     if (!c || nsamples < 1) return cfail(PS3D_E_ARG, "invalid argument");
-    hipFree(c->dx);
-    hipFree(c->dt);
+    (void)hipFree(c->dx);
+    (void)hipFree(c->dt);
     c->dx = nullptr;
     c->dt = nullptr;
     c->ns = 0;
@@ -818,8 +818,8 @@ int ps3d_cnn_timing(ps3d_cnn *c, int reps, float *fwd_ms, float *train_ms) {
     CHK(hipEventSynchronize(e1));
     CHK(hipEventElapsedTime(&ms, e0, e1));
     if (train_ms) *train_ms = ms / reps;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return 0;
 }
 

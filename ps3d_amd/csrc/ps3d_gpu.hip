@@ -110,6 +110,7 @@ struct ps3d_handle {
     std::vector<hipEvent_t> prof_ev;
     std::vector<int> prof_kind;
     size_t prof_used = 0;
+    bool prof_err = false;   // an event could not be created or recorded
 };
 
 // Event pair around the next launch when profiling is on.
@@ -118,16 +119,20 @@ static void prof_begin(ps3d_handle *h, int kind) {
     if (h->prof_used + 2 > h->prof_ev.size()) {
         for (int i = 0; i < 64; ++i) {
             hipEvent_t e;
-            hipEventCreate(&e);
+            if (hipEventCreate(&e) != hipSuccess) {
+                h->prof_err = true;
+                h->prof = false;
+                return;
+            }
             h->prof_ev.push_back(e);
         }
     }
-    hipEventRecord(h->prof_ev[h->prof_used], h->stream);
+    if (hipEventRecord(h->prof_ev[h->prof_used], h->stream) != hipSuccess) h->prof_err = true;
     h->prof_kind.push_back(kind);
 }
 static void prof_end(ps3d_handle *h) {
     if (!h->prof) return;
-    hipEventRecord(h->prof_ev[h->prof_used + 1], h->stream);
+    if (hipEventRecord(h->prof_ev[h->prof_used + 1], h->stream) != hipSuccess) h->prof_err = true;
     h->prof_used += 2;
 }
 
@@ -217,10 +222,10 @@ static ZTabs ztabs(const ps3d_handle *h) {
 
 static void drop_step_graph(ps3d_handle *h) {
     ++h->gen;
-    if (h->sexec) hipGraphExecDestroy(h->sexec);
-    if (h->sgraph) hipGraphDestroy(h->sgraph);
-    if (h->gexec) hipGraphExecDestroy(h->gexec);
-    if (h->ggraph) hipGraphDestroy(h->ggraph);
+    if (h->sexec) (void)hipGraphExecDestroy(h->sexec);
+    if (h->sgraph) (void)hipGraphDestroy(h->sgraph);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->ggraph) (void)hipGraphDestroy(h->ggraph);
     h->sexec = nullptr;
     h->sgraph = nullptr;
     h->gexec = nullptr;
@@ -229,35 +234,35 @@ static void drop_step_graph(ps3d_handle *h) {
 
 static void free_handle(ps3d_handle *h) {
     if (!h) return;
-    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_step_graph(h);
-    hipFree(h->pool);
-    hipFree(h->pres);
-    hipFree(h->delta);
-    hipFree(h->stats_part);
-    hipFree(h->stats_out);
-    hipFree(h->spec_part);
-    hipFree(h->spec_out);
-    hipFree(h->spec_cpart);
-    hipFree(h->spec_cnt);
-    hipFree(h->scal);
-    hipFree(h->party);
-    hipFree(h->partx);
-    hipFree(h->partm);
-    hipFree(h->roll);
-    hipFree(h->diag);
-    hipFree(h->rkx);
-    hipFree(h->rky);
-    hipFree(h->filx);
-    hipFree(h->fily);
-    hipFree(h->filz);
-    hipFree(h->twx);
-    hipFree(h->twy);
-    hipFree(h->twz);
-    for (hipEvent_t e : h->prof_ev) hipEventDestroy(e);
-    if (h->ev0) hipEventDestroy(h->ev0);
-    if (h->ev1) hipEventDestroy(h->ev1);
-    if (h->stream) hipStreamDestroy(h->stream);
+    (void)hipFree(h->pool);
+    (void)hipFree(h->pres);
+    (void)hipFree(h->delta);
+    (void)hipFree(h->stats_part);
+    (void)hipFree(h->stats_out);
+    (void)hipFree(h->spec_part);
+    (void)hipFree(h->spec_out);
+    (void)hipFree(h->spec_cpart);
+    (void)hipFree(h->spec_cnt);
+    (void)hipFree(h->scal);
+    (void)hipFree(h->party);
+    (void)hipFree(h->partx);
+    (void)hipFree(h->partm);
+    (void)hipFree(h->roll);
+    (void)hipFree(h->diag);
+    (void)hipFree(h->rkx);
+    (void)hipFree(h->rky);
+    (void)hipFree(h->filx);
+    (void)hipFree(h->fily);
+    (void)hipFree(h->filz);
+    (void)hipFree(h->twx);
+    (void)hipFree(h->twy);
+    (void)hipFree(h->twz);
+    for (hipEvent_t e : h->prof_ev) (void)hipEventDestroy(e);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
 
@@ -844,7 +849,7 @@ static int write_scal(ps3d_handle *h, int idx, double v) {
 static int ensure_diag(ps3d_handle *h, int rows) {
     if (rows <= h->diag_cap) return 0;
     drop_step_graph(h);
-    hipFree(h->diag);
+    (void)hipFree(h->diag);
     h->diag = nullptr;
     HIPCHK(hipMalloc((void **)&h->diag, sizeof(double) * PS3D_NDIAG * (size_t)rows * h->nbatch));
     h->diag_cap = rows;
@@ -1103,12 +1108,12 @@ static int capture_step(ps3d_handle *h, bool gated, hipGraph_t *graph, hipGraphE
     hipGraph_t gr = nullptr;
     const hipError_t ce = hipStreamEndCapture(h->stream, &gr);
     if (st || ce != hipSuccess) {
-        if (gr) hipGraphDestroy(gr);
+        if (gr) (void)hipGraphDestroy(gr);
         return st ? st : fail(PS3D_E_HIP, std::string("step capture: ") + hipGetErrorString(ce));
     }
     const hipError_t ie = hipGraphInstantiate(exec, gr, nullptr, nullptr, 0);
     if (ie != hipSuccess) {
-        hipGraphDestroy(gr);
+        (void)hipGraphDestroy(gr);
         *exec = nullptr;
         return fail(PS3D_E_HIP, std::string("step graph: ") + hipGetErrorString(ie));
     }
@@ -1194,7 +1199,7 @@ int ps3d_gpu_rk_substep(ps3d_handle *h, int stage, double dt, const double *vdis
     a.rk.vdiss = dv;
     hipError_t e = launch_zrk(h->zl, a, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    hipFree(dv);
+    (void)hipFree(dv);
     if (e != hipSuccess) return fail(PS3D_E_HIP, std::string("rk_substep: ") + hipGetErrorString(e));
     return 0;
 }
@@ -1208,11 +1213,13 @@ int ps3d_gpu_profile_kernels(ps3d_handle *h, int nsteps, double *ms, int *launch
     double zero = 0.0;
     HIPCHK(hipMemcpyAsync(h->scal + SC_STEP, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
     h->prof = true;
+    h->prof_err = false;
     h->prof_used = 0;
     h->prof_kind.clear();
     for (int s = 0; s < nsteps && !st; ++s) st = one_step(h);
     h->prof = false;
     if (st) return st;
+    if (h->prof_err) return fail(PS3D_E_HIP, "profiling events could not be created or recorded");
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int k = 0; k < PS3D_NKIND; ++k) {
         if (ms) ms[k] = 0.0;
@@ -1374,10 +1381,10 @@ constexpr int kSpecMaxBlocks = 1024;
 
 static int spec_buffers(ps3d_handle *h, int nbins) {
     if (nbins <= h->spec_cap && h->spec_part) return 0;
-    hipFree(h->spec_part);
-    hipFree(h->spec_out);
-    hipFree(h->spec_cpart);
-    hipFree(h->spec_cnt);
+    (void)hipFree(h->spec_part);
+    (void)hipFree(h->spec_out);
+    (void)hipFree(h->spec_cpart);
+    (void)hipFree(h->spec_cnt);
     h->spec_part = h->spec_out = nullptr;
     h->spec_cpart = nullptr;
     h->spec_cnt = nullptr;
@@ -1647,7 +1654,7 @@ int ps3d_gpu_fft1d(int kind, int m, int n, double *x) {
                 if (!st && hipStreamSynchronize(h->stream) != hipSuccess) st = fail(PS3D_E_HIP, "fft1d sync");
                 if (!st && hipMemcpy2D(x, w, buf, sizeof(double) * h->g.zs, w, n, hipMemcpyDeviceToHost) != hipSuccess)
                     st = fail(PS3D_E_HIP, "fft1d copy out");
-                hipFree(buf);
+                (void)hipFree(buf);
             }
         }
     } else {
@@ -1664,8 +1671,8 @@ int ps3d_gpu_fft1d(int kind, int m, int n, double *x) {
             const size_t nv = (size_t)p.nx * p.ny * h->g.zs;
             double *buf = nullptr;
             if (hipMalloc((void **)&buf, sizeof(double) * 2 * nv) != hipSuccess) st = fail(PS3D_E_NOMEM, "fft1d alloc");
+            if (!st && hipMemset(buf, 0, sizeof(double) * 2 * nv) != hipSuccess) st = fail(PS3D_E_HIP, "fft1d clear");
             if (!st) {
-                hipMemset(buf, 0, sizeof(double) * 2 * nv);
                 std::vector<double> hb((size_t)p.nx * p.ny * h->g.zs, 0.0);
                 for (int c = 0; c < ncol; ++c)
                     for (int j = 0; j <= n; ++j) {
@@ -1674,20 +1681,22 @@ int ps3d_gpu_fft1d(int kind, int m, int n, double *x) {
                         else if (j >= 1) v = x[c + (size_t)m * (j - 1)];
                         hb[(size_t)c * h->g.zs + j] = v;
                     }
-                hipMemcpy(buf, hb.data(), sizeof(double) * nv, hipMemcpyHostToDevice);
+                if (hipMemcpy(buf, hb.data(), sizeof(double) * nv, hipMemcpyHostToDevice) != hipSuccess)
+                    st = fail(PS3D_E_HIP, "fft1d copy in");
                 ZOpArgs za = zop_args(h, kind == 2 ? ZOP_DCT : ZOP_DST, buf, buf + nv);
-                if (launch_zop(h->zl, za, h->stream) != hipSuccess) st = fail(PS3D_E_HIP, "fft1d z launch");
+                if (!st && launch_zop(h->zl, za, h->stream) != hipSuccess) st = fail(PS3D_E_HIP, "fft1d z launch");
                 if (!st && hipStreamSynchronize(h->stream) != hipSuccess) st = fail(PS3D_E_HIP, "fft1d sync");
+                if (!st && hipMemcpy(hb.data(), buf + nv, sizeof(double) * nv, hipMemcpyDeviceToHost) != hipSuccess)
+                    st = fail(PS3D_E_HIP, "fft1d copy out");
                 if (!st) {
-                    hipMemcpy(hb.data(), buf + nv, sizeof(double) * nv, hipMemcpyDeviceToHost);
                     for (int c = 0; c < ncol; ++c)
                         for (int j = 0; j <= n; ++j) {
                             if (kind == 2) x[c + (size_t)m * j] = hb[(size_t)c * h->g.zs + j];
                             else if (j >= 1) x[c + (size_t)m * (j - 1)] = hb[(size_t)c * h->g.zs + j];
                         }
                 }
-                hipFree(buf);
             }
+            (void)hipFree(buf);   // (a null pointer is a no-op)
         }
     }
     std::string msg = g_err;
@@ -1836,22 +1845,22 @@ struct ps3d_ensemble {
 };
 
 static void drop_graph(ps3d_ensemble *e) {
-    if (e->exec) hipGraphExecDestroy(e->exec);
-    if (e->graph) hipGraphDestroy(e->graph);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
     e->exec = nullptr;
     e->graph = nullptr;
 }
 
 int ps3d_gpu_ensemble_destroy(ps3d_ensemble *e) {
     if (!e) return 0;
-    if (e->s0) hipStreamSynchronize(e->s0);
+    if (e->s0) (void)hipStreamSynchronize(e->s0);
     drop_graph(e);
     for (ps3d_handle *h : e->g) free_handle(h);
-    for (hipEvent_t ev : e->join) hipEventDestroy(ev);
-    if (e->fork) hipEventDestroy(e->fork);
-    if (e->ev0) hipEventDestroy(e->ev0);
-    if (e->ev1) hipEventDestroy(e->ev1);
-    if (e->s0) hipStreamDestroy(e->s0);
+    for (hipEvent_t ev : e->join) (void)hipEventDestroy(ev);
+    if (e->fork) (void)hipEventDestroy(e->fork);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->s0) (void)hipStreamDestroy(e->s0);
     delete e;
     return 0;
 }
@@ -1912,7 +1921,7 @@ static int build_graph(ps3d_ensemble *e) {
     hipGraph_t gr = nullptr;
     hipError_t ce = hipStreamEndCapture(e->s0, &gr);
     if (st) {
-        if (gr) hipGraphDestroy(gr);
+        if (gr) (void)hipGraphDestroy(gr);
         return st;
     }
     if (ce != hipSuccess) return fail(PS3D_E_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
@@ -2075,7 +2084,7 @@ int ps3d_gpu_ensemble_tune(ps3d_ensemble *e, int steps, int *choice) {
         HIPCHK(hipMemcpyAsync(h->scal, bsc, sizeof(double) * SC_NSCAL * nb, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipMemcpyAsync(h->roll, brl, sizeof(double) * h->b_roll * nb, hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
-        hipFree(bk);
+        (void)hipFree(bk);
         if (st) return st;
         if (choice) choice[gi] = best;
     }
